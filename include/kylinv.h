@@ -1,0 +1,179 @@
+/*
+ * kylinv.h -- C ABI of the MI355X (gfx950) Holstein TDVP engine.
+ *
+ * Plain C: opaque handles, int status codes (0 = ok, < 0 = error, message in
+ * kv_last_error), row-major interleaved complex<double> payloads (re, im pairs),
+ * int32 charges and shapes.  No torch or HIP types cross this boundary.
+ *
+ * The reference (YiHeXu517/Kylin-V) has no FFI: its hot path is a set of header-only
+ * C++ templates called by value.  Each entry point below names the template it
+ * replaces.  Tensor conventions follow the reference exactly:
+ *   dirs +1 = In, -1 = Out, every block satisfies sum(dir*qn) == 0
+ *                                             (holstein/tensor.h:20-27, :201-211)
+ *   MPS site   (a, s, b)        dirs {+1,+1,-1}   (holstein/state.h:35-39)
+ *   MPO site   (s', wL, s, wR)  dirs {+1,+1,-1,-1} (holstein/operator.h:35-39)
+ *   left env   (a', w, a)       dirs {+1,-1,-1}, boundary qn (0,0,0)
+ *   right env  (b', w, b)       dirs {-1,+1,+1}, boundary qn (Q,0,Q)
+ *                                             (holstein/tdvp.h:302-311)
+ * Every call is ordered on the context's HIP stream; calls that return host values
+ * synchronise that stream.  One context per host thread.
+ */
+#ifndef KYLINV_H
+#define KYLINV_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kv_ctx kv_ctx;
+typedef struct kv_tensor kv_tensor;
+typedef struct kv_tdvp kv_tdvp;
+
+/* Arithmetic of the effective-Hamiltonian and environment contractions.  Lanczos
+ * scalars, vector algebra, the Krylov exponential and the bond SVD are FP64 always. */
+enum kv_precision {
+    KV_FP64 = 0,   /* complex FP64 on v_mfma_f64_16x16x4f64                        */
+    KV_FP32 = 1    /* complex FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation     */
+};
+
+/* Status codes. */
+enum kv_status {
+    KV_OK = 0,
+    KV_ERR_ARG = -1,        /* bad argument (null handle, rank, size ...)            */
+    KV_ERR_DIRS = -2,       /* contracted legs' directions mismatch
+                               (the reference exits: holstein/product.h:102-110)    */
+    KV_ERR_HIP = -3,        /* HIP runtime error                                     */
+    KV_ERR_SOLVER = -4,     /* rocSOLVER failure (the reference exits on a LAPACK
+                               error: util/allocation.h:219, :229)                  */
+    KV_ERR_FORMAT = -5,     /* bad MPS/MPO file magic or truncated file
+                               (holstein/state.h:134, holstein/operator.h:121)      */
+    KV_ERR_SYM = -6,        /* block breaks charge conservation
+                               (holstein/tensor.h:201-211)                          */
+    KV_ERR_INTERNAL = -7
+};
+
+/* ---- context ------------------------------------------------------------------ */
+int kv_ctx_create(int device, kv_ctx** out);
+int kv_ctx_destroy(kv_ctx* ctx);
+/* Last error message of this context (empty string if none). */
+const char* kv_last_error(const kv_ctx* ctx);
+/* Block until all work queued on the context's stream has finished. */
+int kv_sync(kv_ctx* ctx);
+
+/* ---- block-sparse tensors (holstein/block.h:23-162, holstein/tensor.h:16-212) ---- */
+/* Create a tensor with the given block structure, zero-filled.  qn and shape are
+ * nblocks x rank int arrays.  Blocks with equal qn are rejected (the reference merges
+ * them with push_back_ch, holstein/tensor.h:173-186: pass unique blocks). */
+int kv_tensor_create(kv_ctx* ctx, int rank, const int* dirs, int nblocks,
+                     const int* qn, const int* shape, kv_tensor** out);
+int kv_tensor_destroy(kv_tensor* t);
+/* Structure queries.  Blocks are reported in the tensor's own order (ascending qn). */
+int kv_tensor_rank(const kv_tensor* t, int* rank, int* nblocks, int64_t* nelem);
+int kv_tensor_dirs(const kv_tensor* t, int* dirs);
+int kv_tensor_blocks(const kv_tensor* t, int* qn, int* shape, int64_t* offset);
+/* Copy the payload (nelem complex values, blocks in the tensor's order, each block
+ * row-major) in or out.  data = 2*nelem doubles. */
+int kv_tensor_upload(kv_tensor* t, const double* data);
+int kv_tensor_download(const kv_tensor* t, double* data);
+/* Euclidean norm (holstein/tensor.h:214-223) and <b|a> over blocks of equal qn
+ * (overlap(a, b), holstein/tensor.h:225-241). */
+int kv_tensor_norm(const kv_tensor* t, double* out);
+int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im);
+
+/* ---- contractions ---------------------------------------------------------------- */
+/* product<N1,N2,nc>(m1, m2, T1, T2) (holstein/product.h:61-201).  op = 'N', 'T', 'C'.
+ * 'N' contracts the last nc legs of m1 / first nc legs of m2; 'T'/'C' the first nc
+ * legs of m1 / last nc legs of m2; 'C' conjugates. */
+int kv_product(kv_ctx* ctx, const kv_tensor* m1, const kv_tensor* m2, int nc,
+               char op1, char op2, int precision, kv_tensor** out);
+
+/* Effective Hamiltonian y = L . W1 . W2 . x . R.  nsite = 2 replaces apply_double
+ * (holstein/product.h:478-486, W1 and W2 used); nsite = 1 replaces apply_single
+ * (:547-553, W1 used, W2 ignored); nsite = 0 replaces apply_zero (:540-545, no MPO).
+ * The output has the union block structure of x and H_eff(x). */
+int kv_heff_apply(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                  const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x,
+                  int precision, kv_tensor** y);
+
+/* env_sweep(env, A, W, Left2Right) (holstein/product.h:488-503); with bra != NULL the
+ * bra/ket form env_sweep(env, bra, W, ket) (holstein/product.h:505-521). */
+int kv_env_update(kv_ctx* ctx, const kv_tensor* env, const kv_tensor* bra,
+                  const kv_tensor* W, const kv_tensor* ket, int left_to_right,
+                  int precision, kv_tensor** out);
+
+/* local_krylov<Rank> (holstein/exponential.h:92-190): x <- exp(-/+ i (dt/2) H_eff) x/|x|
+ * in place (the output tensor may gain blocks; the handle is updated).  reverse != 0
+ * uses exp(+i ...).  Writes iterations used, final error estimate and 1 if the
+ * Lanczos error reached tol (0 if the iteration cap was hit). */
+typedef struct kv_krylov_exit {
+    int converged;
+    int iterations;
+    double error;
+    double loss_norm;
+} kv_krylov_exit;
+int kv_krylov_step(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                   const kv_tensor* W2, const kv_tensor* R, kv_tensor* x, double dt,
+                   int reverse, double tol, int max_iter, int precision,
+                   kv_krylov_exit* exit_info);
+
+/* svd_impl<R1,R2>(A, cfg, LeftDir=-1) (holstein/svd.h:296-440): sector SVD in FP64
+ * (rocSOLVER zgesvd), global descending sort, drop entry i (never 0) while
+ * i >= maxdim or s_i <= cutoff; S absorbed into V^dag if left_to_right else into U.
+ * nkept receives the kept bond dimension. */
+int kv_svd_truncate(kv_ctx* ctx, const kv_tensor* A, int r1, double cutoff, int maxdim,
+                    int left_to_right, kv_tensor** left, kv_tensor** right, int* nkept);
+
+/* ---- TDVP driver (holstein/tdvp.h:34-386) ------------------------------------------ */
+typedef struct kv_tdvp_config {
+    double dt;          /* -v (src/kln-tdvp.cpp:41-44)                                 */
+    double cutoff;      /* -t SVD cutoff, default 1e-8 (src/kln-tdvp.cpp:27)           */
+    int maxdim;         /* -D bond cap, default 1000 (src/kln-tdvp.cpp:26)             */
+    double krylov_tol;  /* fixed 1e-8 in the reference (src/kln-tdvp.cpp:27)           */
+    int krylov_maxiter; /* fixed 15 in the reference (src/kln-tdvp.cpp:27)             */
+    int precision;      /* kv_precision of H_eff / environment contractions            */
+} kv_tdvp_config;
+
+/* Takes copies of the MPS (nsites tensors) and MPO; builds the boundary environments
+ * and the right-environment stack (holstein/tdvp.h:300-316). */
+int kv_tdvp_create(kv_ctx* ctx, int nsites, kv_tensor* const* mps, kv_tensor* const* mpo,
+                   const kv_tdvp_config* cfg, kv_tdvp** out);
+int kv_tdvp_destroy(kv_tdvp* s);
+/* One time step dt: TDVP::two_site (holstein/tdvp.h:144-227) or TDVP::one_site
+ * (holstein/tdvp.h:78-141). */
+int kv_tdvp_two_site(kv_tdvp* s);
+int kv_tdvp_one_site(kv_tdvp* s);
+/* A copy of MPS site i (caller destroys it). */
+int kv_tdvp_get_site(kv_tdvp* s, int i, kv_tensor** out);
+/* Bond dimensions (nsites-1 ints), cf. State::max_bond (holstein/state.h:187-200). */
+int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
+/* Per-module timing in seconds, names of the reference's Timer modules
+ * (holstein/tdvp.h:85-214): "Evl-2","Evl-1","Evl-0","SVD","Update","Sweep", plus
+ * "heff_kernel" (GPU time of H_eff contraction kernels, HIP events) and flop counters.
+ * Writes up to n values into out in this order:
+ *   0 Evl-2  1 Evl-1  2 Evl-0  3 SVD  4 Update  5 Sweep  6 heff_gpu_s  7 heff_flop
+ *   8 env_gpu_s  9 env_flop  10 matvecs  11 gemm_gpu_s  12 gemm_flop  13 gemm_launches */
+int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
+int kv_tdvp_reset_stats(kv_tdvp* s);
+
+/* ---- observables (holstein/operator.h:178-198, holstein/state.h:206-227) ------------ */
+/* <bra|O|ket> (Operator::join) and <s1|s2> (overlap(State, State)). */
+int kv_join(kv_ctx* ctx, int nsites, kv_tensor* const* mpo, kv_tensor* const* bra,
+            kv_tensor* const* ket, double* re_im);
+int kv_overlap_states(kv_ctx* ctx, int nsites, kv_tensor* const* s1, kv_tensor* const* s2,
+                      double* re_im);
+
+/* ---- reference binary formats (holstein/state.h:107-157, operator.h:94-143) --------- */
+/* MPS file: INT 111, INT ns, per site INT nb, per block INT shape[3], INT qn[3],
+ * complex<double>[prod(shape)].  MPO: magic 112, rank-4 blocks.  load returns
+ * KV_ERR_FORMAT on a bad magic (the reference exits). */
+int kv_mps_load(kv_ctx* ctx, const char* path, int max_sites, int* nsites, kv_tensor** sites);
+int kv_mpo_load(kv_ctx* ctx, const char* path, int max_sites, int* nsites, kv_tensor** sites);
+int kv_mps_save(kv_ctx* ctx, const char* path, int nsites, kv_tensor* const* sites);
+int kv_mpo_save(kv_ctx* ctx, const char* path, int nsites, kv_tensor* const* sites);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

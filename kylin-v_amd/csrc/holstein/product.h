@@ -1,0 +1,298 @@
+// Executors of the contraction plans: block-sparse product, MPO fold, transposes, the
+// effective-Hamiltonian matvec and the environment update, all on the context stream.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "plan.h"
+#include "tensor.h"
+
+namespace kv {
+
+enum Prec { PREC_F64 = 0, PREC_F32 = 1 };
+
+// Event pairs recorded around contraction kernels; resolved after the host has
+// synchronised the stream anyway (no extra syncs on the hot path).
+struct EventLog {
+    struct Rec {
+        hipEvent_t a, b;
+        double flop;
+        int kind;  // 0 gemm, 1 wfold
+    };
+    std::vector<Rec> pending;
+    std::vector<hipEvent_t> spare;
+    hipEvent_t get() {
+        if (!spare.empty()) {
+            hipEvent_t e = spare.back();
+            spare.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        KV_HIP(hipEventCreate(&e));
+        return e;
+    }
+    // call only after the stream has been synchronised
+    void resolve(KernelStats& st) {
+        for (auto& r : pending) {
+            float ms = 0;
+            KV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+            if (r.kind == 0) {
+                st.gemm_ms += ms;
+                st.gemm_flop += r.flop;
+                st.gemm_launches += 1;
+            } else {
+                st.wfold_ms += ms;
+                st.wfold_flop += r.flop;
+                st.wfold_launches += 1;
+            }
+            spare.push_back(r.a);
+            spare.push_back(r.b);
+        }
+        pending.clear();
+    }
+    ~EventLog() {
+        for (auto& r : pending) {
+            (void)hipEventDestroy(r.a);
+            (void)hipEventDestroy(r.b);
+        }
+        for (auto e : spare) (void)hipEventDestroy(e);
+    }
+};
+
+struct DevProduct {
+    ProductPlan p;
+    DevBuf tiles, groups, segs;
+    void build(Device& dev, ProductPlan plan) {
+        p = std::move(plan);
+        tiles = upload_vec(dev, p.tiles);
+        groups = upload_vec(dev, p.groups);
+        segs = upload_vec(dev, p.segs);
+    }
+    template <class T2>
+    void run(Device& dev, const T2* A, const T2* B, T2* C, double alpha_re = 1.0,
+             double alpha_im = 0.0, EventLog* log = nullptr) const {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (log) {
+            a = log->get();
+            b = log->get();
+            KV_HIP(hipEventRecord(a, dev.stream));
+        }
+        launch_gemm<T2>(dev.stream, p.opA, p.opB, A, B, C, tiles.as<GemmTile>(), (int)p.tiles.size(),
+                        groups.as<GemmGroup>(), segs.as<GemmSeg>(), alpha_re, alpha_im);
+        if (log) {
+            KV_HIP(hipEventRecord(b, dev.stream));
+            log->pending.push_back({a, b, p.flop, 0});
+        }
+    }
+};
+
+struct DevWFold {
+    WFoldPlan p;
+    DevBuf tasks, terms;
+    void build(Device& dev, WFoldPlan plan) {
+        p = std::move(plan);
+        tasks = upload_vec(dev, p.tasks);
+        terms = upload_vec(dev, p.terms);
+    }
+    template <class T2>
+    void run(Device& dev, const T2* in, T2* out, EventLog* log = nullptr) const {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (log) {
+            a = log->get();
+            b = log->get();
+            KV_HIP(hipEventRecord(a, dev.stream));
+        }
+        launch_wfold<T2>(dev.stream, in, out, tasks.as<WTask>(), (int)p.tasks.size(),
+                         terms.as<WTerm>(), p.maxP);
+        if (log) {
+            KV_HIP(hipEventRecord(b, dev.stream));
+            log->pending.push_back({a, b, p.flop, 1});
+        }
+    }
+};
+
+// product<N1,N2,nc>(m1, m2, T1, T2) on the device, FP64 (holstein/product.h:61-201).
+inline std::unique_ptr<Tensor> product(const std::shared_ptr<Device>& dev, const Tensor& A,
+                                       const Tensor& B, int nc, char opA, char opB) {
+    DevProduct dp;
+    dp.build(*dev, plan_product(A.lay, B.lay, nc, opA, opB));
+    auto C = Tensor::create(dev, dp.p.out, false);
+    dp.run<double2>(*dev, A.d(), B.d(), C->d());
+    return C;
+}
+
+// transpose(t, axes) on the device (holstein/tensor.h:244-260).
+inline std::unique_ptr<Tensor> transpose(const std::shared_ptr<Device>& dev, const Tensor& X,
+                                         const Idx& axes) {
+    PermPlan P = plan_perm(X.lay, axes);
+    auto Y = Tensor::create(dev, P.out, false);
+    DevBuf tasks = upload_vec(*dev, P.tasks);
+    launch_perm<double2>(dev->stream, X.d(), Y->d(), tasks.as<PermTask>(), (int)P.tasks.size(),
+                         P.maxsize);
+    dev->sync();  // tasks buffer is released on return
+    return Y;
+}
+
+// Copy X into the (larger) layout `to`, zero elsewhere.
+inline std::unique_ptr<Tensor> relayout(const std::shared_ptr<Device>& dev, const Tensor& X,
+                                        const Layout& to) {
+    if (same_layout(X.lay, to)) return X.clone();
+    auto Y = Tensor::create(dev, to, true);
+    std::vector<Copy2D> cps;
+    int maxrows = 1;
+    for (const auto& b : X.lay.blocks) {
+        const int ob = to.find(b.qn);
+        KV_REQUIRE(ob >= 0 && qn_equal(to.blocks[ob].shape, b.shape, to.rank), ERR_INTERNAL,
+                   "relayout: block missing from target layout");
+        const int last = b.shape[X.lay.rank - 1];
+        Copy2D c{};
+        c.src_off = b.off;
+        c.dst_off = to.blocks[ob].off;
+        c.rows = (int)(b.size / last);
+        c.cols = last;
+        c.src_ld = last;
+        c.dst_ld = last;
+        maxrows = std::max(maxrows, c.rows);
+        cps.push_back(c);
+    }
+    if (!cps.empty()) {
+        DevBuf d = upload_vec(*dev, cps);
+        launch_copy2d<double2>(dev->stream, X.d(), Y->d(), d.as<Copy2D>(), (int)cps.size(),
+                               maxrows, nullptr);
+        dev->sync();
+    }
+    return Y;
+}
+
+// Effective Hamiltonian on one bond / site, built once per Lanczos call
+// (apply_double / apply_single / apply_zero, holstein/product.h:478-553).
+struct Heff {
+    std::shared_ptr<Device> dev;
+    HeffStructure S;
+    Prec prec = PREC_F64;
+    DevProduct s1, s4;
+    DevWFold s2, s3;
+    const Tensor* L = nullptr;
+    const Tensor* R = nullptr;
+    DevBuf L32, R32, x32, y32, t1, t2, t3;
+
+    void build(const std::shared_ptr<Device>& d, int nsite, const Tensor& Lt, Tensor* W1,
+               Tensor* W2, const Tensor& Rt, const Layout& x, Prec p) {
+        dev = d;
+        prec = p;
+        L = &Lt;
+        R = &Rt;
+        const double* w1d = W1 ? W1->host_data().data() : nullptr;
+        const double* w2d = W2 ? W2->host_data().data() : nullptr;
+        S = plan_heff(nsite, Lt.lay, W1 ? &W1->lay : nullptr, w1d, W2 ? &W2->lay : nullptr, w2d,
+                      Rt.lay, x);
+        s1.build(*dev, S.s1);
+        if (nsite >= 1) s2.build(*dev, S.s2);
+        if (nsite == 2) s3.build(*dev, S.s3);
+        s4.build(*dev, S.s4);
+        const size_t es = (p == PREC_F64) ? sizeof(double2) : sizeof(float2);
+        t1 = DevBuf(dev->pool.get(), (size_t)S.s1.out.total * es);
+        if (nsite >= 1) t2 = DevBuf(dev->pool.get(), (size_t)S.s2.out.total * es);
+        if (nsite == 2) t3 = DevBuf(dev->pool.get(), (size_t)S.s3.out.total * es);
+        if (p == PREC_F32) {
+            L32 = DevBuf(dev->pool.get(), (size_t)Lt.lay.total * sizeof(float2));
+            R32 = DevBuf(dev->pool.get(), (size_t)Rt.lay.total * sizeof(float2));
+            x32 = DevBuf(dev->pool.get(), (size_t)S.xl.total * sizeof(float2));
+            y32 = DevBuf(dev->pool.get(), (size_t)S.xl.total * sizeof(float2));
+            launch_f64_to_f32(dev->stream, Lt.d(), L32.as<float2>(), Lt.lay.total);
+            launch_f64_to_f32(dev->stream, Rt.d(), R32.as<float2>(), Rt.lay.total);
+        }
+    }
+    const Layout& layout() const { return S.xl; }
+    double flop() const { return S.flop; }
+
+    template <class T2>
+    void chain(const T2* Ld, const T2* Rd, const T2* x, T2* y, double alpha, EventLog* log) {
+        s1.run<T2>(*dev, Ld, x, t1.as<T2>(), 1.0, 0.0, log);
+        const T2* last = t1.as<T2>();
+        if (S.nsite >= 1) {
+            s2.run<T2>(*dev, last, t2.as<T2>(), log);
+            last = t2.as<T2>();
+        }
+        if (S.nsite == 2) {
+            s3.run<T2>(*dev, last, t3.as<T2>(), log);
+            last = t3.as<T2>();
+        }
+        s4.run<T2>(*dev, last, Rd, y, alpha, 0.0, log);
+    }
+    // y = alpha * H_eff x, x and y FP64 slabs in layout() order
+    void apply(const double2* x, double2* y, double alpha, EventLog* log = nullptr) {
+        if (prec == PREC_F64) {
+            chain<double2>(L->d(), R->d(), x, y, alpha, log);
+        } else {
+            launch_f64_to_f32(dev->stream, x, x32.as<float2>(), S.xl.total);
+            chain<float2>(L32.as<float2>(), R32.as<float2>(), x32.as<float2>(), y32.as<float2>(),
+                          1.0, log);
+            launch_f32_to_f64(dev->stream, y32.as<float2>(), y, alpha, S.xl.total);
+        }
+    }
+};
+
+// env_sweep (holstein/product.h:488-521): L2R  L' = bra^dag . (L . ket . W);
+// R2L: the same after transposing bra/ket (2,1,0) and W (0,3,2,1).
+inline std::unique_ptr<Tensor> env_update(const std::shared_ptr<Device>& dev, const Tensor& env,
+                                          const Tensor* bra, Tensor& W, const Tensor& ket,
+                                          bool l2r, Prec prec, EventLog* log = nullptr,
+                                          double* flop = nullptr) {
+    std::unique_ptr<Tensor> kt, bt;
+    const Tensor* k = &ket;
+    const Tensor* b = bra ? bra : &ket;
+    Layout wl;
+    std::vector<double> wt;
+    const Layout* wlay = &W.lay;
+    const double* wd = W.host_data().data();
+    if (!l2r) {
+        const Idx ax{2, 1, 0, 0, 0, 0};
+        kt = transpose(dev, ket, ax);
+        k = kt.get();
+        if (bra) {
+            bt = transpose(dev, *bra, ax);
+            b = bt.get();
+        } else {
+            b = k;
+        }
+        host_perm(W.lay, wd, Idx{0, 3, 2, 1, 0, 0}, wl, wt);
+        wlay = &wl;
+        wd = wt.data();
+    }
+    EnvStructure E = plan_env_l2r(env.lay, b->lay, *wlay, wd, k->lay);
+    if (flop) *flop += E.flop;
+    DevProduct s1, s3;
+    DevWFold s2;
+    s1.build(*dev, E.s1);
+    s2.build(*dev, E.s2);
+    s3.build(*dev, E.s3);
+    auto out = Tensor::create(dev, E.s3.out, false);
+    if (prec == PREC_F64) {
+        DevBuf t1(dev->pool.get(), (size_t)E.s1.out.total * sizeof(double2));
+        DevBuf t2(dev->pool.get(), (size_t)E.s2.out.total * sizeof(double2));
+        s1.run<double2>(*dev, env.d(), k->d(), t1.as<double2>(), 1.0, 0.0, log);
+        s2.run<double2>(*dev, t1.as<double2>(), t2.as<double2>(), log);
+        s3.run<double2>(*dev, b->d(), t2.as<double2>(), out->d(), 1.0, 0.0, log);
+        dev->sync();
+    } else {
+        DevBuf e32(dev->pool.get(), (size_t)env.lay.total * sizeof(float2));
+        DevBuf k32(dev->pool.get(), (size_t)k->lay.total * sizeof(float2));
+        DevBuf b32(dev->pool.get(), (size_t)b->lay.total * sizeof(float2));
+        DevBuf o32(dev->pool.get(), (size_t)E.s3.out.total * sizeof(float2));
+        DevBuf t1(dev->pool.get(), (size_t)E.s1.out.total * sizeof(float2));
+        DevBuf t2(dev->pool.get(), (size_t)E.s2.out.total * sizeof(float2));
+        launch_f64_to_f32(dev->stream, env.d(), e32.as<float2>(), env.lay.total);
+        launch_f64_to_f32(dev->stream, k->d(), k32.as<float2>(), k->lay.total);
+        launch_f64_to_f32(dev->stream, b->d(), b32.as<float2>(), b->lay.total);
+        s1.run<float2>(*dev, e32.as<float2>(), k32.as<float2>(), t1.as<float2>(), 1.0, 0.0, log);
+        s2.run<float2>(*dev, t1.as<float2>(), t2.as<float2>(), log);
+        s3.run<float2>(*dev, b32.as<float2>(), t2.as<float2>(), o32.as<float2>(), 1.0, 0.0, log);
+        launch_f32_to_f64(dev->stream, o32.as<float2>(), out->d(), 1.0, E.s3.out.total);
+        dev->sync();
+    }
+    return out;
+}
+
+}  // namespace kv

@@ -1,0 +1,274 @@
+// TDVP sweep driver and observables on the device.
+// Follows class TDVP (holstein/tdvp.h:34-386): boundary environments and the right
+// stack (:300-316), two_site (:144-227), one_site (:78-141); Operator::join
+// (holstein/operator.h:178-198) and overlap(State, State) (holstein/state.h:206-227).
+#pragma once
+
+#include <chrono>
+#include <complex>
+#include <memory>
+#include <vector>
+
+#include "exponential.h"
+#include "svd.h"
+
+namespace kv {
+
+struct TdvpConfig {
+    double dt = 0.05;
+    double cutoff = 1e-8;
+    int maxdim = 1000;
+    double krylov_tol = 1e-8;
+    int krylov_maxiter = 15;
+    Prec prec = PREC_F64;
+};
+
+// Wall-clock module timers with the reference's Timer names (holstein/tdvp.h:85-214).
+struct TdvpTimers {
+    double evl2 = 0, evl1 = 0, evl0 = 0, svd = 0, update = 0, sweep = 0;
+};
+
+inline std::unique_ptr<Tensor> left_boundary(const std::shared_ptr<Device>& dev) {
+    Layout l;
+    l.rank = 3;
+    l.dirs = Idx{1, -1, -1, 0, 0, 0};
+    BlockDesc b;
+    b.qn = Idx{0, 0, 0, 0, 0, 0};
+    b.shape = Idx{1, 1, 1, 0, 0, 0};
+    l.blocks.push_back(b);
+    l.finalize();
+    auto t = Tensor::create(dev, l, false);
+    const double one[2] = {1.0, 0.0};
+    t->upload(one);
+    return t;
+}
+
+inline std::unique_ptr<Tensor> right_boundary(const std::shared_ptr<Device>& dev, int Q) {
+    Layout l;
+    l.rank = 3;
+    l.dirs = Idx{-1, 1, 1, 0, 0, 0};
+    BlockDesc b;
+    b.qn = Idx{Q, 0, Q, 0, 0, 0};
+    b.shape = Idx{1, 1, 1, 0, 0, 0};
+    l.blocks.push_back(b);
+    l.finalize();
+    auto t = Tensor::create(dev, l, false);
+    const double one[2] = {1.0, 0.0};
+    t->upload(one);
+    return t;
+}
+
+class Clock {
+  public:
+    Clock() : t0_(std::chrono::steady_clock::now()) {}
+    double s() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0_).count();
+    }
+
+  private:
+    std::chrono::steady_clock::time_point t0_;
+};
+
+class TDVP {
+  public:
+    TDVP(std::shared_ptr<Device> dev, std::vector<std::unique_ptr<Tensor>> mps,
+         std::vector<std::unique_ptr<Tensor>> mpo, const TdvpConfig& cfg)
+        : dev_(std::move(dev)), sol_(new Solver(*dev_)), mps_(std::move(mps)),
+          mpo_(std::move(mpo)), cfg_(cfg) {
+        KV_REQUIRE(mps_.size() == mpo_.size() && mps_.size() >= 2, ERR_ARG,
+                   "TDVP needs matching MPS/MPO of at least two sites");
+        for (auto& w : mpo_) w->host_data();
+        envL_.push_back(left_boundary(dev_));
+        const Layout& last = mps_.back()->lay;
+        KV_REQUIRE(!last.blocks.empty(), ERR_ARG, "empty MPS site");
+        envR_.push_back(right_boundary(dev_, last.blocks[0].qn[2]));
+        for (int i = (int)mps_.size() - 1; i > 0; --i)
+            envR_.push_back(update(*envR_.back(), *mps_[i], *mpo_[i], false));
+    }
+
+    // One time step (holstein/tdvp.h:144-227).
+    void two_site() {
+        const int ns = (int)mps_.size();
+        envR_.pop_back();
+        for (int i = 0; i < ns - 1; ++i) {
+            Clock c;
+            auto mm = product(dev_, *mps_[i], *mps_[i + 1], 1, 'N', 'N');
+            krylov(2, mm, mpo_[i].get(), mpo_[i + 1].get(), false);
+            tm_.evl2 += c.s();
+            Clock c2;
+            SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, true);
+            tm_.svd += c2.s();
+            mps_[i] = std::move(r.left);
+            if (i < ns - 2) {
+                Clock c3;
+                envL_.push_back(update(*envL_.back(), *mps_[i], *mpo_[i], true));
+                tm_.update += c3.s();
+                Clock c4;
+                krylov(1, r.right, mpo_[i + 1].get(), nullptr, true);
+                tm_.evl1 += c4.s();
+                mps_[i + 1] = std::move(r.right);
+                envR_.pop_back();
+            } else {
+                mps_[i + 1] = std::move(r.right);
+            }
+        }
+        for (int i = ns - 1; i > 0; --i) {
+            Clock c;
+            auto mm = product(dev_, *mps_[i - 1], *mps_[i], 1, 'N', 'N');
+            krylov(2, mm, mpo_[i - 1].get(), mpo_[i].get(), false);
+            tm_.evl2 += c.s();
+            Clock c2;
+            SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, false);
+            tm_.svd += c2.s();
+            mps_[i] = std::move(r.right);
+            if (i > 1) {
+                Clock c3;
+                envR_.push_back(update(*envR_.back(), *mps_[i], *mpo_[i], false));
+                tm_.update += c3.s();
+                Clock c4;
+                krylov(1, r.left, mpo_[i - 1].get(), nullptr, true);
+                tm_.evl1 += c4.s();
+                mps_[i - 1] = std::move(r.left);
+                envL_.pop_back();
+            } else {
+                mps_[i - 1] = std::move(r.left);
+            }
+        }
+        Clock c5;
+        envR_.push_back(update(*envR_.back(), *mps_[1], *mpo_[1], false));
+        tm_.update += c5.s();
+    }
+
+    // One time step of one-site TDVP (holstein/tdvp.h:78-141).
+    void one_site() {
+        const int ns = (int)mps_.size();
+        for (int i = 0; i < ns; ++i) {
+            Clock c;
+            krylov(1, mps_[i], mpo_[i].get(), nullptr, false);
+            tm_.evl1 += c.s();
+            if (i < ns - 1) {
+                Clock c2;
+                SvdResult r = svd_truncate(dev_, *sol_, *mps_[i], 2, cfg_.cutoff, cfg_.maxdim, true);
+                tm_.svd += c2.s();
+                mps_[i] = std::move(r.left);
+                Clock c3;
+                envL_.push_back(update(*envL_.back(), *mps_[i], *mpo_[i], true));
+                tm_.update += c3.s();
+                Clock c4;
+                krylov(0, r.right, nullptr, nullptr, true);
+                tm_.evl0 += c4.s();
+                Clock c5;
+                mps_[i + 1] = product(dev_, *r.right, *mps_[i + 1], 1, 'N', 'N');
+                tm_.sweep += c5.s();
+                envR_.pop_back();
+            }
+        }
+        for (int i = ns - 1; i >= 0; --i) {
+            Clock c;
+            krylov(1, mps_[i], mpo_[i].get(), nullptr, false);
+            tm_.evl1 += c.s();
+            if (i > 0) {
+                Clock c2;
+                SvdResult r = svd_truncate(dev_, *sol_, *mps_[i], 1, cfg_.cutoff, cfg_.maxdim, false);
+                tm_.svd += c2.s();
+                mps_[i] = std::move(r.right);
+                Clock c3;
+                envR_.push_back(update(*envR_.back(), *mps_[i], *mpo_[i], false));
+                tm_.update += c3.s();
+                Clock c4;
+                krylov(0, r.left, nullptr, nullptr, true);
+                tm_.evl0 += c4.s();
+                Clock c5;
+                mps_[i - 1] = product(dev_, *mps_[i - 1], *r.left, 1, 'N', 'N');
+                tm_.sweep += c5.s();
+                envL_.pop_back();
+            }
+        }
+    }
+
+    int nsites() const { return (int)mps_.size(); }
+    const Tensor& site(int i) const { return *mps_[i]; }
+    std::vector<int> bond_dims() const {
+        std::vector<int> d;
+        for (int i = 0; i + 1 < (int)mps_.size(); ++i) d.push_back(mps_[i]->lay.leg_dim(2));
+        return d;
+    }
+    const TdvpTimers& timers() const { return tm_; }
+    const LanczosStats& lanczos_stats() const { return ls_; }
+    double env_flop() const { return env_flop_; }
+    Device& device() { return *dev_; }
+    void reset_stats() {
+        tm_ = TdvpTimers();
+        ls_ = LanczosStats();
+        env_flop_ = 0;
+        dev_->stats = KernelStats();
+    }
+    bool timing = true;
+
+  private:
+    std::unique_ptr<Tensor> update(const Tensor& env, const Tensor& A, Tensor& W, bool l2r) {
+        auto out = env_update(dev_, env, nullptr, W, A, l2r, cfg_.prec, timing ? &log_ : nullptr,
+                              &env_flop_);
+        if (timing) log_.resolve(dev_->stats);
+        return out;
+    }
+    void krylov(int nsite, std::unique_ptr<Tensor>& x, Tensor* W1, Tensor* W2, bool reverse) {
+        local_krylov(dev_, nsite, *envL_.back(), W1, W2, *envR_.back(), x, cfg_.dt,
+                     cfg_.krylov_tol, cfg_.krylov_maxiter, reverse, cfg_.prec,
+                     timing ? &log_ : nullptr, &ls_);
+    }
+
+    std::shared_ptr<Device> dev_;
+    std::unique_ptr<Solver> sol_;
+    std::vector<std::unique_ptr<Tensor>> mps_, mpo_;
+    std::vector<std::unique_ptr<Tensor>> envL_, envR_;
+    TdvpConfig cfg_;
+    TdvpTimers tm_;
+    LanczosStats ls_;
+    EventLog log_;
+    double env_flop_ = 0;
+};
+
+// Value of the single 1x1x1 (or 1x1) closing block of a finished environment pass.
+inline std::complex<double> closing_value(Tensor& env) {
+    const auto& h = env.host_data();
+    for (const auto& b : env.lay.blocks) {
+        bool diag = b.qn[0] == b.qn[env.lay.rank - 1];
+        for (int r = 1; r + 1 < env.lay.rank; ++r) diag = diag && b.qn[r] == 0;
+        if (diag && b.size == 1) return {h[2 * b.off], h[2 * b.off + 1]};
+    }
+    return {0.0, 0.0};
+}
+
+// <bra|O|ket> (holstein/operator.h:178-198)
+inline std::complex<double> join(const std::shared_ptr<Device>& dev, std::vector<Tensor*>& mpo,
+                                 const std::vector<const Tensor*>& bra,
+                                 const std::vector<const Tensor*>& ket) {
+    auto env = left_boundary(dev);
+    for (size_t i = 0; i < mpo.size(); ++i)
+        env = env_update(dev, *env, bra[i], *mpo[i], *ket[i], true, PREC_F64);
+    return closing_value(*env);
+}
+
+// <s1|s2> (holstein/state.h:206-227)
+inline std::complex<double> overlap_states(const std::shared_ptr<Device>& dev,
+                                           const std::vector<const Tensor*>& s1,
+                                           const std::vector<const Tensor*>& s2) {
+    Layout l;
+    l.rank = 2;
+    l.dirs = Idx{1, -1, 0, 0, 0, 0};
+    BlockDesc b;
+    b.shape = Idx{1, 1, 0, 0, 0, 0};
+    l.blocks.push_back(b);
+    l.finalize();
+    auto env = Tensor::create(dev, l, false);
+    const double one[2] = {1.0, 0.0};
+    env->upload(one);
+    for (size_t i = 0; i < s1.size(); ++i) {
+        auto lm = product(dev, *env, *s2[i], 1, 'N', 'N');
+        env = product(dev, *s1[i], *lm, 2, 'C', 'N');
+    }
+    return closing_value(*env);
+}
+
+}  // namespace kv

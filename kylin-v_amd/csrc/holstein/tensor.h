@@ -1,0 +1,84 @@
+// Device-resident U(1) block-sparse tensor: a Layout (sorted block table) plus one FP64
+// complex slab in HBM.  Replaces Block<Rank>/Tensor<Rank> (holstein/block.h:23-162,
+// holstein/tensor.h:16-212), whose blocks are separate host mallocs.
+#pragma once
+
+#include <memory>
+#include <vector>
+
+#include "../kv_kernels.h"
+#include "../kv_runtime.h"
+#include "layout.h"
+
+namespace kv {
+
+struct Tensor {
+    std::shared_ptr<Device> dev;
+    Layout lay;
+    DevBuf buf;                 // double2[lay.total]
+    std::vector<double> host;   // host mirror (interleaved re, im), valid when host_valid
+    bool host_valid = false;
+
+    double2* d() const { return buf.as<double2>(); }
+
+    static std::unique_ptr<Tensor> create(const std::shared_ptr<Device>& dev, Layout lay,
+                                          bool zero = true) {
+        std::unique_ptr<Tensor> t(new Tensor);
+        t->dev = dev;
+        t->lay = std::move(lay);
+        t->buf = DevBuf(dev->pool.get(), (size_t)t->lay.total * sizeof(double2));
+        if (zero && t->lay.total) launch_zero(dev->stream, t->buf.get(), t->buf.bytes());
+        return t;
+    }
+    void upload(const double* data) {
+        if (lay.total)
+            KV_HIP(hipMemcpyAsync(buf.get(), data, (size_t)lay.total * sizeof(double2),
+                                  hipMemcpyHostToDevice, dev->stream));
+        host.assign(data, data + 2 * (size_t)lay.total);
+        host_valid = true;
+        dev->sync();
+    }
+    void download(double* data) const {
+        if (lay.total)
+            KV_HIP(hipMemcpyAsync(data, buf.get(), (size_t)lay.total * sizeof(double2),
+                                  hipMemcpyDeviceToHost, dev->stream));
+        dev->sync();
+    }
+    // Host copy of the payload (downloads once; MPO tensors keep theirs from upload).
+    const std::vector<double>& host_data() {
+        if (!host_valid) {
+            host.resize(2 * (size_t)lay.total);
+            download(host.data());
+            host_valid = true;
+        }
+        return host;
+    }
+    void invalidate_host() { host_valid = false; }
+    std::unique_ptr<Tensor> clone() const {
+        auto t = create(dev, lay, false);
+        if (lay.total)
+            KV_HIP(hipMemcpyAsync(t->buf.get(), buf.get(), (size_t)lay.total * sizeof(double2),
+                                  hipMemcpyDeviceToDevice, dev->stream));
+        if (host_valid) {
+            t->host = host;
+            t->host_valid = true;
+        }
+        return t;
+    }
+};
+
+// Host-to-device upload of a plan's descriptor array into a fresh device buffer.  The
+// copy is ordered on the context stream (the pooled buffer may still be read by earlier
+// work) and completed before returning, so the host vector may be freed afterwards.
+template <class T>
+inline DevBuf upload_vec(Device& dev, const std::vector<T>& v) {
+    DevBuf b(dev.pool.get(), v.size() * sizeof(T));
+    if (!v.empty()) {
+        KV_HIP(hipMemcpyAsync(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
+                              dev.stream));
+        dev.sync();
+    }
+    return b;
+}
+
+}  // namespace kv

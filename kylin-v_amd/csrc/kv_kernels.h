@@ -1,0 +1,52 @@
+// Launcher entry points of the gfx950 kernels.  Work descriptors live in kv_types.h.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kv_types.h"
+
+namespace kv {
+
+template <class T2>
+void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2* C,
+                 const GemmTile* tiles, int ntiles, const GemmGroup* groups,
+                 const GemmSeg* segs, double alpha_re, double alpha_im);
+
+template <class T2>
+void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,
+                  const WTerm* terms, int maxP);
+
+template <class T2>
+void launch_perm(hipStream_t st, const T2* in, T2* out, const PermTask* tasks, int ntasks,
+                 int64_t maxsize);
+
+template <class T2>
+void launch_copy2d(hipStream_t st, const T2* src, T2* dst, const Copy2D* cps, int ncopies,
+                   int maxrows, const double* scale);
+
+// FP64 vector algebra over flat slabs of n complex values.
+// bytes must be a multiple of 4
+void launch_zero(hipStream_t st, void* p, size_t bytes);
+// out[0] = sum conj(a) b  (double2); scratch >= 2*1024 doubles
+void launch_dotc(hipStream_t st, const double2* a, const double2* b, int64_t n,
+                 double2* out, double* scratch);
+// out[0] = sum |a|^2
+void launch_nrm2sq(hipStream_t st, const double2* a, int64_t n, double* out, double* scratch);
+// y -= c[0] * x   (c on the device)
+void launch_axpy_neg_dev(hipStream_t st, double2* y, const double2* x, const double2* c,
+                         int64_t n);
+// dst = src / sqrt(nsq[0])  (nsq on the device)
+void launch_scale_invsqrt_dev(hipStream_t st, double2* dst, const double2* src,
+                              const double* nsq, int64_t n);
+// dst = src * s
+void launch_scale(hipStream_t st, double2* dst, const double2* src, double s, int64_t n);
+// out = sum_i c[i] * V[i*n + e],  c on the device
+void launch_lincomb(hipStream_t st, double2* out, const double2* V, const double2* c, int nv,
+                    int64_t n);
+void launch_f64_to_f32(hipStream_t st, const double2* src, float2* dst, int64_t n);
+// dst = s * src
+void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s, int64_t n);
+
+}  // namespace kv

@@ -1,0 +1,551 @@
+// gfx950 (CDNA4) kernels of the Holstein TDVP engine.
+//
+//  * zgemm_grouped: complex GEMM over a device-resident list of U(1) block pairs.  One
+//    launch covers every output block of a contraction stage; a group sums its K
+//    segments (the block pairs that land on the same output block) inside one tile,
+//    so there are no atomics and results are deterministic.  Complex products run as
+//    four real MFMAs (re.re - im.im, re.im + im.re) on v_mfma_f64_16x16x4f64 or
+//    v_mfma_f32_16x16x4f32; 64x64 complex tile per 256-thread workgroup, 4 waves in
+//    a 2x2 arrangement of 32x32 complex wave tiles, operands staged through LDS.
+//  * wfold: applies the sparse MPO tensor as a gather-sum over contiguous slices.
+//  * perm / copy2d: block transposes and sector gather/scatter.
+//  * FP64 vector algebra for the Lanczos recurrence (deterministic two-pass reductions).
+#include "kv_kernels.h"
+#include "kv_runtime.h"
+
+namespace kv {
+
+typedef double f64x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+template <class T> struct Cx;
+template <> struct Cx<double> {
+    using T2 = double2;
+    using acc = f64x4;
+    __device__ static inline acc mfma(double a, double b, acc c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+    // v_mfma_f64_16x16x4f64: lane l, accumulator item v holds D[l/16 + 4v][l%16]
+    // (measured on gfx950 with one-hot operands)
+    __device__ static inline int out_row(int lk, int v) { return lk + 4 * v; }
+};
+template <> struct Cx<float> {
+    using T2 = float2;
+    using acc = f32x4;
+    __device__ static inline acc mfma(float a, float b, acc c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+    // v_mfma_f32_16x16x4f32: lane l, accumulator item v holds D[4(l/16) + v][l%16]
+    __device__ static inline int out_row(int lk, int v) { return 4 * lk + v; }
+};
+
+template <class T2> struct ScalarOf;
+template <> struct ScalarOf<double2> { using T = double; };
+template <> struct ScalarOf<float2> { using T = float; };
+
+// ------------------------------------------------------------------------------------
+// grouped complex GEMM
+// ------------------------------------------------------------------------------------
+template <class T, int OPA, int OPB>
+__global__ __launch_bounds__(256) void zgemm_grouped_kernel(
+    const typename Cx<T>::T2* __restrict__ A, const typename Cx<T>::T2* __restrict__ B,
+    typename Cx<T>::T2* __restrict__ C, const GemmTile* __restrict__ tiles,
+    const GemmGroup* __restrict__ groups, const GemmSeg* __restrict__ segs, T alpha_re,
+    T alpha_im) {
+    using T2 = typename Cx<T>::T2;
+    using acc_t = typename Cx<T>::acc;
+    constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
+    __shared__ T2 As[BK][BM + 1];
+    __shared__ T2 Bs[BK][BN + 1];
+
+    const GemmTile tile = tiles[blockIdx.x];
+    const GemmGroup g = groups[tile.group];
+    const int m0 = tile.tm * BM, n0 = tile.tn * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    acc_t cre[2][2], cim[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            cre[i][j] = acc_t{0, 0, 0, 0};
+            cim[i][j] = acc_t{0, 0, 0, 0};
+        }
+
+    for (int s = 0; s < g.seg_count; ++s) {
+        const GemmSeg sg = segs[g.seg_begin + s];
+        const T2* __restrict__ a = A + sg.a_off;
+        const T2* __restrict__ b = B + sg.b_off;
+        const int K = sg.k;
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            // stage A (BM x BK) as As[k][m]
+#pragma unroll
+            for (int r = 0; r < (BM * BK) / 256; ++r) {
+                const int e = tid + r * 256;
+                int mm, kk;
+                if (OPA == 0) {
+                    mm = e / BK;
+                    kk = e % BK;
+                } else {
+                    kk = e / BM;
+                    mm = e % BM;
+                }
+                const int gm = m0 + mm, gk = k0 + kk;
+                T2 v;
+                v.x = 0;
+                v.y = 0;
+                if (gm < g.m && gk < K) {
+                    v = (OPA == 0) ? a[(int64_t)gm * sg.lda + gk] : a[(int64_t)gk * sg.lda + gm];
+                    if (OPA == 2) v.y = -v.y;
+                }
+                As[kk][mm] = v;
+            }
+            // stage B (BK x BN) as Bs[k][n]
+#pragma unroll
+            for (int r = 0; r < (BN * BK) / 256; ++r) {
+                const int e = tid + r * 256;
+                int nn, kk;
+                if (OPB == 0) {
+                    kk = e / BN;
+                    nn = e % BN;
+                } else {
+                    nn = e / BK;
+                    kk = e % BK;
+                }
+                const int gn = n0 + nn, gk = k0 + kk;
+                T2 v;
+                v.x = 0;
+                v.y = 0;
+                if (gn < g.n && gk < K) {
+                    v = (OPB == 0) ? b[(int64_t)gk * sg.ldb + gn] : b[(int64_t)gn * sg.ldb + gk];
+                    if (OPB == 2) v.y = -v.y;
+                }
+                Bs[kk][nn] = v;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kq = 0; kq < BK; kq += 4) {
+                T2 av[2], bv[2];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
+                        cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
+                        cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
+                        cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
+                    }
+            }
+            __syncthreads();
+        }
+    }
+    // epilogue: MFMA output layout per type (Cx<T>::out_row)
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + Cx<T>::out_row(lk, v);
+                const int col = n0 + wn + j * 16 + lr;
+                if (row < g.m && col < g.n) {
+                    const T re = cre[i][j][v], im = cim[i][j][v];
+                    T2 o;
+                    o.x = alpha_re * re - alpha_im * im;
+                    o.y = alpha_re * im + alpha_im * re;
+                    C[g.c_off + (int64_t)row * g.ldc + col] = o;
+                }
+            }
+}
+
+template <class T2>
+void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2* C,
+                 const GemmTile* tiles, int ntiles, const GemmGroup* groups,
+                 const GemmSeg* segs, double alpha_re, double alpha_im) {
+    using T = typename ScalarOf<T2>::T;
+    if (ntiles <= 0) return;
+    const dim3 grid(ntiles), block(256);
+    const T ar = (T)alpha_re, ai = (T)alpha_im;
+#define KV_GEMM_CASE(OA, OB)                                                              \
+    if (opA == OA && opB == OB) {                                                        \
+        hipLaunchKernelGGL((zgemm_grouped_kernel<T, OA, OB>), grid, block, 0, st, A, B, C, \
+                           tiles, groups, segs, ar, ai);                                   \
+        KV_HIP(hipGetLastError());                                                        \
+        return;                                                                           \
+    }
+    KV_GEMM_CASE(0, 0)
+    KV_GEMM_CASE(0, 1)
+    KV_GEMM_CASE(0, 2)
+    KV_GEMM_CASE(1, 0)
+    KV_GEMM_CASE(1, 1)
+    KV_GEMM_CASE(1, 2)
+    KV_GEMM_CASE(2, 0)
+    KV_GEMM_CASE(2, 1)
+    KV_GEMM_CASE(2, 2)
+#undef KV_GEMM_CASE
+    throw Error(ERR_ARG, "launch_gemm: bad op codes");
+}
+
+template void launch_gemm<double2>(hipStream_t, int, int, const double2*, const double2*,
+                                   double2*, const GemmTile*, int, const GemmGroup*,
+                                   const GemmSeg*, double, double);
+template void launch_gemm<float2>(hipStream_t, int, int, const float2*, const float2*,
+                                  float2*, const GemmTile*, int, const GemmGroup*,
+                                  const GemmSeg*, double, double);
+
+// ------------------------------------------------------------------------------------
+// MPO fold
+// ------------------------------------------------------------------------------------
+template <class T2>
+__global__ __launch_bounds__(256) void wfold_kernel(const T2* __restrict__ in,
+                                                    T2* __restrict__ out,
+                                                    const WTask* __restrict__ tasks,
+                                                    const WTerm* __restrict__ terms) {
+    using T = typename ScalarOf<T2>::T;
+    const WTask tk = tasks[blockIdx.x];
+    for (int p = blockIdx.y; p < tk.P; p += gridDim.y) {
+        for (int q = threadIdx.x; q < tk.Q; q += blockDim.x) {
+            T re = 0, im = 0;
+            for (int t = 0; t < tk.term_count; ++t) {
+                const WTerm w = terms[tk.term_begin + t];
+                const T2 v = in[w.in_base + (int64_t)p * w.in_pstride + q];
+                const T wr = (T)w.re, wi = (T)w.im;
+                re += wr * v.x - wi * v.y;
+                im += wr * v.y + wi * v.x;
+            }
+            T2 o;
+            o.x = re;
+            o.y = im;
+            out[tk.out_base + (int64_t)p * tk.out_pstride + q] = o;
+        }
+    }
+}
+
+template <class T2>
+void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,
+                  const WTerm* terms, int maxP) {
+    if (ntasks <= 0) return;
+    const int gy = maxP < 1 ? 1 : (maxP < 64 ? maxP : 64);
+    hipLaunchKernelGGL((wfold_kernel<T2>), dim3(ntasks, gy), dim3(256), 0, st, in, out, tasks,
+                       terms);
+    KV_HIP(hipGetLastError());
+}
+template void launch_wfold<double2>(hipStream_t, const double2*, double2*, const WTask*, int,
+                                    const WTerm*, int);
+template void launch_wfold<float2>(hipStream_t, const float2*, float2*, const WTask*, int,
+                                   const WTerm*, int);
+
+// ------------------------------------------------------------------------------------
+// block permutation
+// ------------------------------------------------------------------------------------
+template <class T2>
+__global__ __launch_bounds__(256) void perm_kernel(const T2* __restrict__ in,
+                                                   T2* __restrict__ out,
+                                                   const PermTask* __restrict__ tasks) {
+    const PermTask tk = tasks[blockIdx.x];
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < tk.size;
+         e += (int64_t)gridDim.y * blockDim.x) {
+        int64_t rem = e, src = 0;
+        for (int r = tk.rank - 1; r >= 0; --r) {
+            const int64_t idx = rem % tk.oshape[r];
+            rem /= tk.oshape[r];
+            src += idx * tk.istride[r];
+        }
+        out[tk.out_off + e] = in[tk.in_off + src];
+    }
+}
+
+template <class T2>
+void launch_perm(hipStream_t st, const T2* in, T2* out, const PermTask* tasks, int ntasks,
+                 int64_t maxsize) {
+    if (ntasks <= 0 || maxsize <= 0) return;
+    int64_t gy = (maxsize + 255) / 256;
+    if (gy > 1024) gy = 1024;
+    hipLaunchKernelGGL((perm_kernel<T2>), dim3(ntasks, (unsigned)gy), dim3(256), 0, st, in,
+                       out, tasks);
+    KV_HIP(hipGetLastError());
+}
+template void launch_perm<double2>(hipStream_t, const double2*, double2*, const PermTask*, int,
+                                   int64_t);
+template void launch_perm<float2>(hipStream_t, const float2*, float2*, const PermTask*, int,
+                                  int64_t);
+
+// ------------------------------------------------------------------------------------
+// 2-D strided copies with optional diagonal scaling
+// ------------------------------------------------------------------------------------
+template <class T2>
+__global__ __launch_bounds__(256) void copy2d_kernel(const T2* __restrict__ src,
+                                                     T2* __restrict__ dst,
+                                                     const Copy2D* __restrict__ cps,
+                                                     const double* __restrict__ scale) {
+    using T = typename ScalarOf<T2>::T;
+    const Copy2D c = cps[blockIdx.x];
+    for (int r = blockIdx.y; r < c.rows; r += gridDim.y) {
+        for (int q = threadIdx.x; q < c.cols; q += blockDim.x) {
+            T2 v = src[c.src_off + (int64_t)r * c.src_ld + q];
+            if (c.scale_mode == 1) {
+                const T f = (T)scale[c.scale_off + r];
+                v.x *= f;
+                v.y *= f;
+            } else if (c.scale_mode == 2) {
+                const T f = (T)scale[c.scale_off + q];
+                v.x *= f;
+                v.y *= f;
+            }
+            dst[c.dst_off + (int64_t)r * c.dst_ld + q] = v;
+        }
+    }
+}
+
+template <class T2>
+void launch_copy2d(hipStream_t st, const T2* src, T2* dst, const Copy2D* cps, int ncopies,
+                   int maxrows, const double* scale) {
+    if (ncopies <= 0 || maxrows <= 0) return;
+    const int gy = maxrows < 256 ? maxrows : 256;
+    hipLaunchKernelGGL((copy2d_kernel<T2>), dim3(ncopies, gy), dim3(256), 0, st, src, dst, cps,
+                       scale);
+    KV_HIP(hipGetLastError());
+}
+template void launch_copy2d<double2>(hipStream_t, const double2*, double2*, const Copy2D*, int,
+                                     int, const double*);
+
+// ------------------------------------------------------------------------------------
+// FP64 vector algebra
+// ------------------------------------------------------------------------------------
+constexpr int RED_BLOCKS = 1024;
+
+__global__ __launch_bounds__(256) void zero_kernel(uint32_t* __restrict__ p, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+        p[e] = 0u;
+}
+
+// Zero fill by a kernel (not hipMemsetAsync), so the call stays safe inside captured graphs.
+void launch_zero(hipStream_t st, void* p, size_t bytes) {
+    if (!bytes) return;
+    const int64_t n = (int64_t)(bytes / 4);
+    int64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    hipLaunchKernelGGL(zero_kernel, dim3((unsigned)b), dim3(256), 0, st, (uint32_t*)p, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void dotc_partial_kernel(const double2* __restrict__ a,
+                                                           const double2* __restrict__ b,
+                                                           int64_t n, double* __restrict__ part) {
+    __shared__ double sr[256], si[256];
+    double re = 0, im = 0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double2 x = a[e], y = b[e];
+        re += x.x * y.x + x.y * y.y;  // conj(x) * y
+        im += x.x * y.y - x.y * y.x;
+    }
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sr[0];
+        part[2 * blockIdx.x + 1] = si[0];
+    }
+}
+
+__global__ __launch_bounds__(256) void reduce2_final_kernel(const double* __restrict__ part,
+                                                            int nb, double* __restrict__ out,
+                                                            int ncomp) {
+    __shared__ double s0[256], s1[256];
+    double a = 0, b = 0;
+    for (int i = threadIdx.x; i < nb; i += 256) {
+        a += part[ncomp * i];
+        if (ncomp == 2) b += part[ncomp * i + 1];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            s0[threadIdx.x] += s0[threadIdx.x + s];
+            s1[threadIdx.x] += s1[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        if (ncomp == 2) out[1] = s1[0];
+    }
+}
+
+static int red_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > RED_BLOCKS) b = RED_BLOCKS;
+    if (b < 1) b = 1;
+    return (int)b;
+}
+
+void launch_dotc(hipStream_t st, const double2* a, const double2* b, int64_t n, double2* out,
+                 double* scratch) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(dotc_partial_kernel, dim3(nb), dim3(256), 0, st, a, b, n, scratch);
+    KV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, scratch, nb,
+                       reinterpret_cast<double*>(out), 2);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void nrm2sq_partial_kernel(const double2* __restrict__ a,
+                                                             int64_t n, double* __restrict__ part) {
+    __shared__ double sr[256];
+    double re = 0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double2 x = a[e];
+        re += x.x * x.x + x.y * x.y;
+    }
+    sr[threadIdx.x] = re;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sr[threadIdx.x] += sr[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sr[0];
+}
+
+void launch_nrm2sq(hipStream_t st, const double2* a, int64_t n, double* out, double* scratch) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(nrm2sq_partial_kernel, dim3(nb), dim3(256), 0, st, a, n, scratch);
+    KV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, scratch, nb, out, 1);
+    KV_HIP(hipGetLastError());
+}
+
+static unsigned ew_blocks(int64_t n) {
+    int64_t b = (n + 255) / 256;
+    if (b > 4096) b = 4096;
+    if (b < 1) b = 1;
+    return (unsigned)b;
+}
+
+__global__ __launch_bounds__(256) void axpy_neg_dev_kernel(double2* __restrict__ y,
+                                                           const double2* __restrict__ x,
+                                                           const double2* __restrict__ c,
+                                                           int64_t n) {
+    const double2 cc = c[0];
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double2 v = x[e];
+        double2 o = y[e];
+        o.x -= cc.x * v.x - cc.y * v.y;
+        o.y -= cc.x * v.y + cc.y * v.x;
+        y[e] = o;
+    }
+}
+
+void launch_axpy_neg_dev(hipStream_t st, double2* y, const double2* x, const double2* c,
+                         int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(axpy_neg_dev_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, y, x, c, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void scale_invsqrt_dev_kernel(double2* __restrict__ dst,
+                                                                const double2* __restrict__ src,
+                                                                const double* __restrict__ nsq,
+                                                                int64_t n) {
+    const double f = 1.0 / sqrt(nsq[0]);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        double2 v = src[e];
+        v.x *= f;
+        v.y *= f;
+        dst[e] = v;
+    }
+}
+
+void launch_scale_invsqrt_dev(hipStream_t st, double2* dst, const double2* src,
+                              const double* nsq, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(scale_invsqrt_dev_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, dst, src,
+                       nsq, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void scale_kernel(double2* __restrict__ dst,
+                                                    const double2* __restrict__ src, double s,
+                                                    int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        double2 v = src[e];
+        v.x *= s;
+        v.y *= s;
+        dst[e] = v;
+    }
+}
+
+void launch_scale(hipStream_t st, double2* dst, const double2* src, double s, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(scale_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, dst, src, s, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void lincomb_kernel(double2* __restrict__ out,
+                                                      const double2* __restrict__ V,
+                                                      const double2* __restrict__ c, int nv,
+                                                      int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        double re = 0, im = 0;
+        for (int i = 0; i < nv; ++i) {
+            const double2 v = V[(int64_t)i * n + e];
+            const double2 ci = c[i];
+            re += ci.x * v.x - ci.y * v.y;
+            im += ci.x * v.y + ci.y * v.x;
+        }
+        out[e] = make_double2(re, im);
+    }
+}
+
+void launch_lincomb(hipStream_t st, double2* out, const double2* V, const double2* c, int nv,
+                    int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(lincomb_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, out, V, c, nv, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void f64_to_f32_kernel(const double2* __restrict__ s,
+                                                         float2* __restrict__ d, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double2 v = s[e];
+        d[e] = make_float2((float)v.x, (float)v.y);
+    }
+}
+
+void launch_f64_to_f32(hipStream_t st, const double2* src, float2* dst, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, src, dst, n);
+    KV_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(256) void f32_to_f64_kernel(const float2* __restrict__ s,
+                                                         double2* __restrict__ d, double f,
+                                                         int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const float2 v = s[e];
+        d[e] = make_double2(f * (double)v.x, f * (double)v.y);
+    }
+}
+
+void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(f32_to_f64_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, src, dst, s, n);
+    KV_HIP(hipGetLastError());
+}
+
+}  // namespace kv

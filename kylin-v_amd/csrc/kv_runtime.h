@@ -1,0 +1,180 @@
+// Device runtime for the Holstein TDVP engine: errors, a caching device allocator,
+// the per-context stream and HIP-event timers.  gfx950 only.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <map>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "kv_error.h"
+
+namespace kv {
+
+#define KV_HIP(call)                                                                   \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            throw ::kv::Error(::kv::ERR_HIP, std::string(#call) + ": " +               \
+                                                 hipGetErrorString(e_));               \
+    } while (0)
+
+// Caching allocator.  hipMalloc/hipFree synchronise the device and cost tens of
+// microseconds, so buffers of the sweep (Krylov vectors, intermediates, environment
+// slabs) are recycled by size class.  Size classes: 256 B granularity below 1 MiB,
+// 1 MiB granularity above.
+class Pool {
+  public:
+    explicit Pool(int device) : device_(device) {}
+    ~Pool() { release_all(); }
+    Pool(const Pool&) = delete;
+    Pool& operator=(const Pool&) = delete;
+
+    static size_t size_class(size_t bytes) {
+        if (bytes == 0) bytes = 1;
+        const size_t g = bytes < (1u << 20) ? 256 : (1u << 20);
+        return (bytes + g - 1) / g * g;
+    }
+    void* get(size_t bytes) {
+        const size_t sz = size_class(bytes);
+        auto it = free_.find(sz);
+        if (it != free_.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            cached_ -= sz;
+            return p;
+        }
+        void* p = nullptr;
+        hipError_t e = hipMalloc(&p, sz);
+        if (e != hipSuccess) {
+            // release cached blocks and retry once
+            (void)hipGetLastError();
+            release_cached();
+            KV_HIP(hipMalloc(&p, sz));
+        }
+        live_ += sz;
+        return p;
+    }
+    void put(void* p, size_t bytes) {
+        if (!p) return;
+        const size_t sz = size_class(bytes);
+        free_[sz].push_back(p);
+        cached_ += sz;
+    }
+    void release_cached() {
+        for (auto& kvp : free_) {
+            for (void* p : kvp.second) {
+                (void)hipFree(p);
+                live_ -= kvp.first;
+            }
+            kvp.second.clear();
+        }
+        cached_ = 0;
+    }
+    void release_all() { release_cached(); }
+    size_t live_bytes() const { return live_; }
+
+  private:
+    int device_;
+    std::map<size_t, std::vector<void*>> free_;
+    size_t live_ = 0, cached_ = 0;
+};
+
+// Owning device buffer returned to the pool on destruction.
+class DevBuf {
+  public:
+    DevBuf() = default;
+    DevBuf(Pool* pool, size_t bytes) : pool_(pool), bytes_(bytes) {
+        ptr_ = bytes ? pool->get(bytes) : nullptr;
+    }
+    ~DevBuf() { reset(); }
+    DevBuf(DevBuf&& o) noexcept : pool_(o.pool_), ptr_(o.ptr_), bytes_(o.bytes_) {
+        o.ptr_ = nullptr;
+        o.bytes_ = 0;
+    }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        if (this != &o) {
+            reset();
+            pool_ = o.pool_;
+            ptr_ = o.ptr_;
+            bytes_ = o.bytes_;
+            o.ptr_ = nullptr;
+            o.bytes_ = 0;
+        }
+        return *this;
+    }
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    void reset() {
+        if (ptr_ && pool_) pool_->put(ptr_, bytes_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    void* get() const { return ptr_; }
+    template <class T> T* as() const { return static_cast<T*>(ptr_); }
+    size_t bytes() const { return bytes_; }
+
+  private:
+    Pool* pool_ = nullptr;
+    void* ptr_ = nullptr;
+    size_t bytes_ = 0;
+};
+
+// Counters accumulated by the contraction executors.
+struct KernelStats {
+    double gemm_flop = 0, gemm_ms = 0;
+    double wfold_flop = 0, wfold_ms = 0;
+    long gemm_launches = 0, wfold_launches = 0;
+};
+
+struct Device {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    std::unique_ptr<Pool> pool;
+    bool timing = false;  // record HIP events around contraction kernels
+    KernelStats stats;
+    explicit Device(int dev) : id(dev) {
+        KV_HIP(hipSetDevice(dev));
+        KV_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        pool.reset(new Pool(dev));
+    }
+    ~Device() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            pool.reset();
+            (void)hipStreamDestroy(stream);
+        }
+    }
+    void sync() { KV_HIP(hipStreamSynchronize(stream)); }
+};
+
+// Times a region of stream work with a pair of HIP events recorded on the stream the
+// kernels run on.  ms() synchronises on the end event.
+class EventTimer {
+  public:
+    EventTimer() {
+        KV_HIP(hipEventCreate(&a_));
+        KV_HIP(hipEventCreate(&b_));
+    }
+    ~EventTimer() {
+        (void)hipEventDestroy(a_);
+        (void)hipEventDestroy(b_);
+    }
+    void start(hipStream_t s) { KV_HIP(hipEventRecord(a_, s)); }
+    void stop(hipStream_t s) { KV_HIP(hipEventRecord(b_, s)); }
+    float ms() {
+        KV_HIP(hipEventSynchronize(b_));
+        float t = 0;
+        KV_HIP(hipEventElapsedTime(&t, a_, b_));
+        return t;
+    }
+
+  private:
+    hipEvent_t a_{}, b_{};
+};
+
+}  // namespace kv

@@ -1,0 +1,209 @@
+"""Holstein exciton-phonon chain as a U(1) MPO + initial product MPS for the engine.
+
+H = J sum_i (|i><i+1| + h.c.) + sum_{i,m} w_m b+_{im} b_{im} + sum_{i,m} g_{im} n_i (b_{im} + b+_{im})
+
+Charge conventions are the reference's (MPS (a,s,b) dirs (+1,+1,-1), holstein/state.h:35-39;
+MPO (s',wL,s,wR) dirs (+1,+1,-1,-1), holstein/operator.h:35-39).  Site layouts:
+
+  "merged"  one site per Holstein site, s = (n_ex, k_1..k_M); charge = n_ex
+  "pp"      the reference's projected purification (holstein/local.h:3-86,
+            holstein/lattice.h:83-339): exciton site, then per mode a physical and an
+            auxiliary site, each basis state its own sector, aux starting full
+  "split"   exciton site then one charge-0 site per mode
+
+The MPO is a channel automaton: 'start'/'done' channels plus one open channel per term
+crossing the bond, carrying the charge that term has raised so far; channels of equal
+charge form one sector of the MPO leg.  This stands in for kln-holstein's MPO sums
+(holstein/lattice.h:183-339), which build the same operator by MPO products and SVD
+compression.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def _boson(L):
+    b = np.diag(np.sqrt(np.arange(1.0, L)), 1)
+    return b, b.T.copy()
+
+
+class Site:
+    def __init__(self, charges, ops):
+        self.q = np.asarray(charges, dtype=np.int64)
+        self.ops = {k: np.asarray(v, dtype=np.complex128) for k, v in ops.items()}
+        self.d = len(self.q)
+        self.sec = {}
+        for i, c in enumerate(self.q):
+            self.sec.setdefault(int(c), []).append(i)
+
+    def dq(self, name):
+        o = self.ops[name]
+        ii, jj = np.nonzero(o)
+        vals = {int(self.q[i] - self.q[j]) for i, j in zip(ii, jj)}
+        if len(vals) > 1:
+            raise ValueError(f"operator {name} has no definite charge")
+        return vals.pop() if vals else 0
+
+
+def _exciton():
+    cd = np.array([[0, 0], [1, 0]], dtype=float)
+    return Site([0, 1], {"I": np.eye(2), "cdag": cd, "c": cd.T, "n": np.diag([0.0, 1.0])})
+
+
+def chain(N, omegas, gs, Ls, J, layout="merged"):
+    """Returns (sites, terms, init, exciton_site_index) for a Holstein chain."""
+    omegas, Ls = list(omegas), list(Ls)
+    M = len(omegas)
+    g = np.asarray(gs, dtype=float)
+    g = np.full((N, M), float(g)) if g.ndim == 0 else g.reshape(N, M)
+    sites, terms, init, ex = [], [], [], []
+
+    def hop(a, b):
+        terms.append((J, [(a, "cdag"), (b, "c")]))
+        terms.append((J, [(a, "c"), (b, "cdag")]))
+
+    if layout == "merged":
+        P = int(np.prod(Ls))
+        e = _exciton()
+        ops = {"I": np.eye(2 * P), "cdag": np.kron(e.ops["cdag"], np.eye(P)),
+               "c": np.kron(e.ops["c"], np.eye(P)), "n": np.kron(e.ops["n"], np.eye(P))}
+        for m, L in enumerate(Ls):
+            b, bd = _boson(L)
+            left, right = np.eye(int(np.prod(Ls[:m]))), np.eye(int(np.prod(Ls[m + 1:])))
+            ops[f"nb{m}"] = np.kron(np.eye(2), np.kron(np.kron(left, bd @ b), right))
+            ops[f"nx{m}"] = np.kron(e.ops["n"], np.kron(np.kron(left, b + bd), right))
+        st = Site([0] * P + [1] * P, ops)
+        for i in range(N):
+            sites.append(st)
+            ex.append(i)
+            init.append(P if i == 0 else 0)
+            for m in range(M):
+                terms.append((omegas[m], [(i, f"nb{m}")]))
+                terms.append((g[i, m], [(i, f"nx{m}")]))
+        for i in range(N - 1):
+            hop(i, i + 1)
+    elif layout in ("split", "pp"):
+        e = _exciton()
+        stride = 1 + (2 * M if layout == "pp" else M)
+        for i in range(N):
+            base = stride * i
+            sites.append(e)
+            ex.append(base)
+            init.append(1 if i == 0 else 0)
+            for m, L in enumerate(Ls):
+                b, bd = _boson(L)
+                if layout == "split":
+                    sites.append(Site([0] * L, {"I": np.eye(L), "nb": bd @ b, "x": b + bd}))
+                    init.append(0)
+                    p = base + 1 + m
+                    terms.append((omegas[m], [(p, "nb")]))
+                    terms.append((g[i, m], [(base, "n"), (p, "x")]))
+                else:
+                    up = np.diag(np.ones(L - 1), -1)
+                    sites.append(Site(list(range(L)), {"I": np.eye(L), "bdag": bd, "b": b,
+                                                       "nb": bd @ b}))
+                    sites.append(Site(list(range(L)), {"I": np.eye(L), "araise": up,
+                                                       "alower": up.T}))
+                    init += [0, L - 1]
+                    p, a = base + 1 + 2 * m, base + 2 + 2 * m
+                    terms.append((omegas[m], [(p, "nb")]))
+                    terms.append((g[i, m], [(base, "n"), (p, "bdag"), (a, "alower")]))
+                    terms.append((g[i, m], [(base, "n"), (p, "b"), (a, "araise")]))
+        for i in range(N - 1):
+            hop(stride * i, stride * (i + 1))
+    else:
+        raise ValueError(f"unknown layout {layout}")
+    return sites, terms, init, ex
+
+
+def mpo_blocks(sites, terms):
+    """MPO tensors as lists of {qn: block} dicts with dirs (+1,+1,-1,-1)."""
+    ns = len(sites)
+    bonds = []
+    for k in range(ns + 1):
+        ch = []
+        if k < ns:
+            ch.append(("start", 0))
+        if k > 0:
+            ch.append(("done", 0))
+        for t, (_, ops) in enumerate(terms):
+            if len(ops) > 1 and ops[0][0] < k <= ops[-1][0]:
+                ch.append((t, sum(sites[s].dq(o) for s, o in ops if s < k)))
+        bonds.append(ch)
+    out = []
+    for k, st in enumerate(sites):
+        cl, cr = bonds[k], bonds[k + 1]
+        il = {c[0]: j for j, c in enumerate(cl)}
+        ir = {c[0]: j for j, c in enumerate(cr)}
+        W = np.zeros((st.d, len(cl), st.d, len(cr)), dtype=np.complex128)
+        for name in ("start", "done"):
+            if name in il and name in ir:
+                W[:, il[name], :, ir[name]] += st.ops["I"]
+        for t, (coef, ops) in enumerate(terms):
+            first, last = ops[0][0], ops[-1][0]
+            if not first <= k <= last:
+                continue
+            o = st.ops[dict(ops).get(k, "I")]
+            src = il["start"] if k == first else il[t]
+            dst = ir["done"] if k == last else ir[t]
+            W[:, src, :, dst] += (coef if k == first else 1.0) * o
+        qL = {}
+        for j, c in enumerate(cl):
+            qL.setdefault(c[1], []).append(j)
+        qR = {}
+        for j, c in enumerate(cr):
+            qR.setdefault(c[1], []).append(j)
+        blocks = {}
+        for qp, ip in st.sec.items():
+            for ql, jl in qL.items():
+                for qs, is_ in st.sec.items():
+                    for qr, jr in qR.items():
+                        blk = W[np.ix_(ip, jl, is_, jr)]
+                        if np.any(blk != 0):
+                            assert qp + ql == qs + qr
+                            blocks[(qp, ql, qs, qr)] = blk
+        out.append(blocks)
+    return out
+
+
+def product_state_blocks(sites, init):
+    """Product MPS with bond qn = running charge (holstein/lattice.h:83-124)."""
+    out, q = [], 0
+    for st, idx in zip(sites, init):
+        qs = int(st.q[idx])
+        blk = np.zeros((1, len(st.sec[qs]), 1), dtype=np.complex128)
+        blk[0, st.sec[qs].index(idx), 0] = 1.0
+        out.append({(q, qs, q + qs): blk})
+        q += qs
+    return out
+
+
+class HolsteinChain:
+    """Device-side MPO, initial MPS and observable MPOs of one chain."""
+
+    def __init__(self, N, omegas, gs, Ls, J, layout="merged", ctx=None):
+        from . import Tensor, default_context
+        self.ctx = ctx or default_context()
+        self.N, self.layout = N, layout
+        self.sites, self.terms, self.init, self.ex = chain(N, omegas, gs, Ls, J, layout)
+        self.J = J
+        self._T = Tensor
+        self.mpo = self._up(mpo_blocks(self.sites, self.terms), (1, 1, -1, -1))
+        self.psi0 = self._up(product_state_blocks(self.sites, self.init), (1, 1, -1))
+        self._density = {}
+
+    def _up(self, blist, dirs):
+        return [self._T.from_blocks(dirs, b, self.ctx) for b in blist]
+
+    def density(self, i):
+        """n on the exciton of Holstein site i (the reference's Op/Density_j)."""
+        if i not in self._density:
+            self._density[i] = self._up(mpo_blocks(self.sites, [(1.0, [(self.ex[i], "n")])]),
+                                        (1, 1, -1, -1))
+        return self._density[i]
+
+    def hopping(self, i):
+        """J (|i><i+1| + h.c.): the bond-energy observable."""
+        a, b = self.ex[i], self.ex[i + 1]
+        terms = [(self.J, [(a, "cdag"), (b, "c")]), (self.J, [(a, "c"), (b, "cdag")])]
+        return self._up(mpo_blocks(self.sites, terms), (1, 1, -1, -1))

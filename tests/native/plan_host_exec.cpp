@@ -1,0 +1,213 @@
+// Test harness (CPU): executes the engine's contraction plans with naive host loops so
+// the planners (holstein/plan.h), the Krylov exponential coefficients (herm_eig.h) and
+// the truncation rule (svd_rule.h) can be compared with the oracle without a GPU.
+// Not part of the product: the product runs the same plans with HIP kernels.
+//
+// usage: plan_host_exec <mode> <in.bin> <out.bin>
+//   modes: heff0 heff1 heff2 (inputs L [W1 [W2]] R x), env_l2r / env_r2l (env W A),
+//          product:<nc><opA><opB> (A B), krycoef:<reverse> (alpha/beta list), trunc
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../kylin-v_amd/csrc/holstein/herm_eig.h"
+#include "../../kylin-v_amd/csrc/holstein/plan.h"
+#include "../../kylin-v_amd/csrc/holstein/svd_rule.h"
+
+using namespace kv;
+using cd = std::complex<double>;
+
+struct HT {
+    Layout lay;
+    std::vector<double> d;  // interleaved
+};
+
+static FILE* fin;
+static int rdi() {
+    int32_t v;
+    if (fread(&v, 4, 1, fin) != 1) exit(3);
+    return v;
+}
+static HT read_tensor() {
+    HT t;
+    t.lay.rank = rdi();
+    for (int r = 0; r < t.lay.rank; ++r) t.lay.dirs[r] = rdi();
+    int nb = rdi();
+    std::vector<std::vector<double>> data(nb);
+    std::vector<BlockDesc> bl(nb);
+    for (int b = 0; b < nb; ++b) {
+        for (int r = 0; r < t.lay.rank; ++r) bl[b].qn[r] = rdi();
+        int64_t s = 1;
+        for (int r = 0; r < t.lay.rank; ++r) {
+            bl[b].shape[r] = rdi();
+            s *= bl[b].shape[r];
+        }
+        data[b].resize(2 * s);
+        if (fread(data[b].data(), 8, 2 * s, fin) != (size_t)(2 * s)) exit(3);
+    }
+    t.lay.blocks = bl;
+    t.lay.finalize();
+    t.d.assign(2 * t.lay.total, 0.0);
+    for (int b = 0; b < nb; ++b) {
+        int i = t.lay.find(bl[b].qn);
+        memcpy(&t.d[2 * t.lay.blocks[i].off], data[b].data(), 8 * data[b].size());
+    }
+    return t;
+}
+static void write_tensor(FILE* f, const HT& t) {
+    auto wi = [&](int v) { fwrite(&v, 4, 1, f); };
+    wi(t.lay.rank);
+    for (int r = 0; r < t.lay.rank; ++r) wi(t.lay.dirs[r]);
+    wi((int)t.lay.blocks.size());
+    for (auto& b : t.lay.blocks) {
+        for (int r = 0; r < t.lay.rank; ++r) wi(b.qn[r]);
+        for (int r = 0; r < t.lay.rank; ++r) wi(b.shape[r]);
+        fwrite(&t.d[2 * b.off], 8, 2 * b.size, f);
+    }
+}
+static const cd* C(const HT& t) { return reinterpret_cast<const cd*>(t.d.data()); }
+static cd* C(HT& t) { return reinterpret_cast<cd*>(t.d.data()); }
+
+static void exec_gemm(const ProductPlan& P, const cd* A, const cd* B, cd* Cc, cd alpha) {
+    for (const auto& g : P.groups)
+        for (int m = 0; m < g.m; ++m)
+            for (int n = 0; n < g.n; ++n) {
+                cd acc = 0;
+                for (int s = 0; s < g.seg_count; ++s) {
+                    const GemmSeg& sg = P.segs[g.seg_begin + s];
+                    for (int k = 0; k < sg.k; ++k) {
+                        cd a = P.opA == 0 ? A[sg.a_off + (int64_t)m * sg.lda + k]
+                                          : A[sg.a_off + (int64_t)k * sg.lda + m];
+                        if (P.opA == 2) a = std::conj(a);
+                        cd b = P.opB == 0 ? B[sg.b_off + (int64_t)k * sg.ldb + n]
+                                          : B[sg.b_off + (int64_t)n * sg.ldb + k];
+                        if (P.opB == 2) b = std::conj(b);
+                        acc += a * b;
+                    }
+                }
+                Cc[g.c_off + (int64_t)m * g.ldc + n] = alpha * acc;
+            }
+}
+static void exec_wfold(const WFoldPlan& F, const cd* in, cd* out) {
+    for (const auto& t : F.tasks)
+        for (int p = 0; p < t.P; ++p)
+            for (int q = 0; q < t.Q; ++q) {
+                cd acc = 0;
+                for (int i = 0; i < t.term_count; ++i) {
+                    const WTerm& w = F.terms[t.term_begin + i];
+                    acc += cd(w.re, w.im) * in[w.in_base + (int64_t)p * w.in_pstride + q];
+                }
+                out[t.out_base + (int64_t)p * t.out_pstride + q] = acc;
+            }
+}
+static HT alloc(const Layout& l) {
+    HT t;
+    t.lay = l;
+    t.d.assign(2 * l.total, 0.0);
+    return t;
+}
+static HT perm(const HT& x, Idx axes) {
+    HT o;
+    host_perm(x.lay, x.d.data(), axes, o.lay, o.d);
+    return o;
+}
+static HT env_l2r(const HT& env, const HT& W, const HT& A) {
+    EnvStructure E = plan_env_l2r(env.lay, A.lay, W.lay, W.d.data(), A.lay);
+    HT t1 = alloc(E.s1.out), t2 = alloc(E.s2.out), o = alloc(E.s3.out);
+    exec_gemm(E.s1, C(env), C(A), C(t1), 1.0);
+    exec_wfold(E.s2, C(t1), C(t2));
+    exec_gemm(E.s3, C(A), C(t2), C(o), 1.0);
+    return o;
+}
+
+int main(int argc, char** argv) {
+    if (argc < 4) return 2;
+    std::string mode = argv[1];
+    fin = fopen(argv[2], "rb");
+    FILE* fo = fopen(argv[3], "wb");
+    try {
+        if (mode.rfind("heff", 0) == 0) {
+            int ns = mode[4] - '0';
+            HT L = read_tensor();
+            HT W1, W2;
+            if (ns >= 1) W1 = read_tensor();
+            if (ns == 2) W2 = read_tensor();
+            HT R = read_tensor(), x = read_tensor();
+            HeffStructure H = plan_heff(ns, L.lay, ns >= 1 ? &W1.lay : nullptr, W1.d.data(),
+                                        ns == 2 ? &W2.lay : nullptr, W2.d.data(), R.lay, x.lay);
+            HT xl = alloc(H.xl);
+            for (auto& b : x.lay.blocks) {
+                int i = H.xl.find(b.qn);
+                memcpy(&xl.d[2 * H.xl.blocks[i].off], &x.d[2 * b.off], 16 * b.size);
+            }
+            HT t1 = alloc(H.s1.out), t2, t3, y = alloc(H.xl);
+            exec_gemm(H.s1, C(L), C(xl), C(t1), 1.0);
+            const HT* last = &t1;
+            if (ns >= 1) {
+                t2 = alloc(H.s2.out);
+                exec_wfold(H.s2, C(t1), C(t2));
+                last = &t2;
+            }
+            if (ns == 2) {
+                t3 = alloc(H.s3.out);
+                exec_wfold(H.s3, C(t2), C(t3));
+                last = &t3;
+            }
+            exec_gemm(H.s4, C(*last), C(R), C(y), 1.0);
+            write_tensor(fo, y);
+        } else if (mode == "env_l2r" || mode == "env_r2l") {
+            HT env = read_tensor(), W = read_tensor(), A = read_tensor();
+            if (mode == "env_r2l") {
+                A = perm(A, Idx{2, 1, 0, 0, 0, 0});
+                W = perm(W, Idx{0, 3, 2, 1, 0, 0});
+            }
+            write_tensor(fo, env_l2r(env, W, A));
+        } else if (mode.rfind("product:", 0) == 0) {
+            int nc = mode[8] - '0';
+            HT A = read_tensor(), B = read_tensor();
+            ProductPlan P = plan_product(A.lay, B.lay, nc, mode[9], mode[10]);
+            HT o = alloc(P.out);
+            exec_gemm(P, C(A), C(B), C(o), 1.0);
+            write_tensor(fo, o);
+        } else if (mode.rfind("krycoef:", 0) == 0) {
+            bool rev = mode[8] == '1';
+            int n = rdi();
+            std::vector<cd> al(n), be(n > 0 ? n - 1 : 0);
+            for (int i = 0; i < n; ++i) {
+                double v[2];
+                if (fread(v, 8, 2, fin) != 2) exit(3);
+                al[i] = cd(v[0], v[1]);
+            }
+            for (int i = 0; i + 1 < n; ++i) {
+                double v[2];
+                if (fread(v, 8, 2, fin) != 2) exit(3);
+                be[i] = cd(v[0], v[1]);
+            }
+            auto c = krylov_coefficients(al, be, rev);
+            fwrite(c.data(), 16, c.size(), fo);
+        } else if (mode == "trunc") {
+            int np = rdi();
+            std::vector<std::vector<double>> sv(np);
+            for (int p = 0; p < np; ++p) {
+                int n = rdi();
+                sv[p].resize(n);
+                if (fread(sv[p].data(), 8, n, fin) != (size_t)n) exit(3);
+            }
+            double cut;
+            if (fread(&cut, 8, 1, fin) != 1) exit(3);
+            int md = rdi();
+            auto k = truncation_counts(sv, cut, md);
+            fwrite(k.data(), 4, k.size(), fo);
+        } else {
+            return 2;
+        }
+    } catch (const Error& e) {
+        fprintf(stderr, "kv error %d: %s\n", e.code, e.what());
+        return 1;
+    }
+    fclose(fo);
+    return 0;
+}
