@@ -150,10 +150,12 @@ int kv_tdvp_get_site(kv_tdvp* s, int i, kv_tensor** out);
 int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
 /* Per-module timing in seconds, names of the reference's Timer modules
  * (holstein/tdvp.h:85-214): "Evl-2","Evl-1","Evl-0","SVD","Update","Sweep", plus
- * "heff_kernel" (GPU time of H_eff contraction kernels, HIP events) and flop counters.
- * Writes up to n values into out in this order:
- *   0 Evl-2  1 Evl-1  2 Evl-0  3 SVD  4 Update  5 Sweep  6 heff_gpu_s  7 heff_flop
- *   8 env_gpu_s  9 env_flop  10 matvecs  11 gemm_gpu_s  12 gemm_flop  13 gemm_launches */
+ * GPU time (HIP events on the context stream) and algorithmic flops of the H_eff matvec
+ * kernels and of the environment-update kernels.  Writes up to n values in this order:
+ *   0 Evl-2  1 Evl-1  2 Evl-0  3 SVD  4 Update  5 Sweep  (wall seconds)
+ *   6 heff_gpu_s  7 heff_flop  8 env_gpu_s  9 env_flop  10 matvecs
+ *   11 heff_gemm_gpu_s  12 heff_gemm_flop  13 heff_gemm_launches
+ *   14 heff_wfold_gpu_s 15 heff_wfold_flop 16 heff_wfold_launches */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

@@ -368,7 +368,8 @@ def save_mpo(path, sites):
 
 
 STAT_NAMES = ["Evl-2", "Evl-1", "Evl-0", "SVD", "Update", "Sweep", "heff_gpu_s", "heff_flop",
-              "reserved", "env_flop", "matvecs", "gemm_gpu_s", "gemm_flop", "gemm_launches"]
+              "env_gpu_s", "env_flop", "matvecs", "heff_gemm_gpu_s", "heff_gemm_flop",
+              "heff_gemm_launches", "heff_wfold_gpu_s", "heff_wfold_flop", "heff_wfold_launches"]
 
 
 class TDVP:

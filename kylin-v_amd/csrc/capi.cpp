@@ -345,12 +345,15 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims) {
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
     if (!s || !s->s || !out) return KV_ERR_ARG;
     const kv::TdvpTimers& t = s->s->timers();
-    const kv::KernelStats& k = s->s->device().stats;
-    const double v[14] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
-                          (k.gemm_ms + k.wfold_ms) * 1e-3, s->s->lanczos_stats().heff_flop,
-                          0.0, s->s->env_flop(), (double)s->s->lanczos_stats().matvecs,
-                          k.gemm_ms * 1e-3, k.gemm_flop, (double)k.gemm_launches};
-    for (int i = 0; i < n && i < 14; ++i) out[i] = v[i];
+    const kv::KernelStats& h = s->s->heff_kernels();
+    const kv::KernelStats& e = s->s->env_kernels();
+    const double v[17] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
+                          (h.gemm_ms + h.wfold_ms) * 1e-3, h.gemm_flop + h.wfold_flop,
+                          (e.gemm_ms + e.wfold_ms) * 1e-3, e.gemm_flop + e.wfold_flop,
+                          (double)s->s->lanczos_stats().matvecs,
+                          h.gemm_ms * 1e-3, h.gemm_flop, (double)h.gemm_launches,
+                          h.wfold_ms * 1e-3, h.wfold_flop, (double)h.wfold_launches};
+    for (int i = 0; i < n && i < 17; ++i) out[i] = v[i];
     return KV_OK;
 }
 

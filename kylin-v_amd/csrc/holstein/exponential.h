@@ -37,7 +37,7 @@ inline KrylovExit local_krylov(const std::shared_ptr<Device>& dev, int nsite, co
                                Tensor* W1, Tensor* W2, const Tensor& R,
                                std::unique_ptr<Tensor>& x, double dt, double tol, int max_iter,
                                bool reverse, Prec prec, EventLog* log = nullptr,
-                               LanczosStats* stats = nullptr) {
+                               LanczosStats* stats = nullptr, KernelStats* kstats = nullptr) {
     Heff H;
     H.build(dev, nsite, L, W1, W2, R, x->lay, prec);
     const Layout& xl = H.layout();
@@ -80,7 +80,7 @@ inline KrylovExit local_krylov(const std::shared_ptr<Device>& dev, int nsite, co
         launch_nrm2sq(D.stream, hv.as<double2>(), n, s + 4, scratch.as<double>());
         KV_HIP(hipMemcpyAsync(host, s, 5 * sizeof(double), hipMemcpyDeviceToHost, D.stream));
         D.sync();
-        if (log) log->resolve(D.stats);
+        if (log) log->resolve(kstats ? *kstats : D.stats);
         if (k > 0) bet_up.emplace_back(host[0], host[1]);
         alp.emplace_back(host[2], host[3]);
         const double nm = std::sqrt(host[4]);

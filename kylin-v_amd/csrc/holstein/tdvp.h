@@ -6,6 +6,8 @@
 
 #include <chrono>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -78,6 +80,8 @@ class TDVP {
         KV_REQUIRE(mps_.size() == mpo_.size() && mps_.size() >= 2, ERR_ARG,
                    "TDVP needs matching MPS/MPO of at least two sites");
         for (auto& w : mpo_) w->host_data();
+        const char* v = std::getenv("KV_VERBOSE");
+        verbose_ = v && v[0] == '1';
         envL_.push_back(left_boundary(dev_));
         const Layout& last = mps_.back()->lay;
         KV_REQUIRE(!last.blocks.empty(), ERR_ARG, "empty MPS site");
@@ -94,10 +98,15 @@ class TDVP {
             Clock c;
             auto mm = product(dev_, *mps_[i], *mps_[i + 1], 1, 'N', 'N');
             krylov(2, mm, mpo_[i].get(), mpo_[i + 1].get(), false);
-            tm_.evl2 += c.s();
+            const double te = c.s();
+            tm_.evl2 += te;
             Clock c2;
             SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, true);
-            tm_.svd += c2.s();
+            const double ts = c2.s();
+            tm_.svd += ts;
+            if (verbose_)
+                std::fprintf(stderr, "[kv] sweep L2R bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d\n",
+                             i, i + 1, te, last_iters_, ts, r.kept, r.total);
             mps_[i] = std::move(r.left);
             if (i < ns - 2) {
                 Clock c3;
@@ -116,10 +125,15 @@ class TDVP {
             Clock c;
             auto mm = product(dev_, *mps_[i - 1], *mps_[i], 1, 'N', 'N');
             krylov(2, mm, mpo_[i - 1].get(), mpo_[i].get(), false);
-            tm_.evl2 += c.s();
+            const double te = c.s();
+            tm_.evl2 += te;
             Clock c2;
             SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, false);
-            tm_.svd += c2.s();
+            const double ts = c2.s();
+            tm_.svd += ts;
+            if (verbose_)
+                std::fprintf(stderr, "[kv] sweep R2L bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d\n",
+                             i - 1, i, te, last_iters_, ts, r.kept, r.total);
             mps_[i] = std::move(r.right);
             if (i > 1) {
                 Clock c3;
@@ -195,13 +209,16 @@ class TDVP {
     }
     const TdvpTimers& timers() const { return tm_; }
     const LanczosStats& lanczos_stats() const { return ls_; }
+    const KernelStats& heff_kernels() const { return heff_k_; }
+    const KernelStats& env_kernels() const { return env_k_; }
     double env_flop() const { return env_flop_; }
     Device& device() { return *dev_; }
     void reset_stats() {
         tm_ = TdvpTimers();
         ls_ = LanczosStats();
         env_flop_ = 0;
-        dev_->stats = KernelStats();
+        heff_k_ = KernelStats();
+        env_k_ = KernelStats();
     }
     bool timing = true;
 
@@ -209,13 +226,14 @@ class TDVP {
     std::unique_ptr<Tensor> update(const Tensor& env, const Tensor& A, Tensor& W, bool l2r) {
         auto out = env_update(dev_, env, nullptr, W, A, l2r, cfg_.prec, timing ? &log_ : nullptr,
                               &env_flop_);
-        if (timing) log_.resolve(dev_->stats);
+        if (timing) log_.resolve(env_k_);
         return out;
     }
     void krylov(int nsite, std::unique_ptr<Tensor>& x, Tensor* W1, Tensor* W2, bool reverse) {
-        local_krylov(dev_, nsite, *envL_.back(), W1, W2, *envR_.back(), x, cfg_.dt,
-                     cfg_.krylov_tol, cfg_.krylov_maxiter, reverse, cfg_.prec,
-                     timing ? &log_ : nullptr, &ls_);
+        KrylovExit ex = local_krylov(dev_, nsite, *envL_.back(), W1, W2, *envR_.back(), x, cfg_.dt,
+                                     cfg_.krylov_tol, cfg_.krylov_maxiter, reverse, cfg_.prec,
+                                     timing ? &log_ : nullptr, &ls_, &heff_k_);
+        last_iters_ = ex.iterations;
     }
 
     std::shared_ptr<Device> dev_;
@@ -226,7 +244,10 @@ class TDVP {
     TdvpTimers tm_;
     LanczosStats ls_;
     EventLog log_;
+    KernelStats heff_k_, env_k_;
     double env_flop_ = 0;
+    bool verbose_ = false;
+    int last_iters_ = 0;
 };
 
 // Value of the single 1x1x1 (or 1x1) closing block of a finished environment pass.

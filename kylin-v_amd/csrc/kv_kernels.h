@@ -26,6 +26,10 @@ template <class T2>
 void launch_copy2d(hipStream_t st, const T2* src, T2* dst, const Copy2D* cps, int ncopies,
                    int maxrows, const double* scale);
 
+// dst[c*ld_dst + r] = conj(src[r*ld_src + c]), src is rows x cols
+void launch_ctranspose(hipStream_t st, const double2* src, double2* dst, int rows, int cols,
+                       int ld_src, int ld_dst);
+
 // FP64 vector algebra over flat slabs of n complex values.
 // bytes must be a multiple of 4
 void launch_zero(hipStream_t st, void* p, size_t bytes);

@@ -316,6 +316,35 @@ void launch_copy2d(hipStream_t st, const T2* src, T2* dst, const Copy2D* cps, in
 template void launch_copy2d<double2>(hipStream_t, const double2*, double2*, const Copy2D*, int,
                                      int, const double*);
 
+// dst[c*ld_dst + r] = conj(src[r*ld_src + c]) for r < rows, c < cols (32x32 LDS tiles)
+__global__ __launch_bounds__(256) void ctranspose_kernel(const double2* __restrict__ src,
+                                                         double2* __restrict__ dst, int rows,
+                                                         int cols, int ld_src, int ld_dst) {
+    __shared__ double2 t[32][33];
+    const int r0 = blockIdx.y * 32, c0 = blockIdx.x * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int i = ty; i < 32; i += 8) {
+        const int r = r0 + i, c = c0 + tx;
+        if (r < rows && c < cols) t[i][tx] = src[(int64_t)r * ld_src + c];
+    }
+    __syncthreads();
+    for (int i = ty; i < 32; i += 8) {
+        const int c = c0 + i, r = r0 + tx;
+        if (r < rows && c < cols) {
+            const double2 v = t[tx][i];
+            dst[(int64_t)c * ld_dst + r] = make_double2(v.x, -v.y);
+        }
+    }
+}
+
+void launch_ctranspose(hipStream_t st, const double2* src, double2* dst, int rows, int cols,
+                       int ld_src, int ld_dst) {
+    if (rows <= 0 || cols <= 0) return;
+    hipLaunchKernelGGL(ctranspose_kernel, dim3((cols + 31) / 32, (rows + 31) / 32), dim3(256), 0,
+                       st, src, dst, rows, cols, ld_src, ld_dst);
+    KV_HIP(hipGetLastError());
+}
+
 // ------------------------------------------------------------------------------------
 // FP64 vector algebra
 // ------------------------------------------------------------------------------------

@@ -207,3 +207,60 @@ class HolsteinChain:
         a, b = self.ex[i], self.ex[i + 1]
         terms = [(self.J, [(a, "cdag"), (b, "c")]), (self.J, [(a, "c"), (b, "cdag")])]
         return self._up(mpo_blocks(self.sites, terms), (1, 1, -1, -1))
+
+
+def synthetic_state_blocks(sites, D, seed=0, decay=1.0, Q=1):
+    """Seeded random right-canonical MPS of total charge Q with every bond filled up to D.
+
+    Used as the benchmark input at the steady-state bond dimension (a run from the product
+    state needs hundreds of steps to fill the bonds).  Bond charges q of the left part lie
+    in [0, Q]; D is split evenly over the charge sectors and capped by the dimension
+    available.  Columns are weighted by exp(-decay * phonon number) so the state sits at
+    low phonon occupation like the physical one, then orthonormalised by QR.
+    """
+    rng = np.random.default_rng(seed)
+    ns = len(sites)
+    out = [None] * ns
+    right = {Q: 1}
+    for i in reversed(range(ns)):
+        st = sites[i]
+        nph = np.zeros(st.d)
+        for name, o in st.ops.items():
+            if name.startswith("nb"):
+                nph += np.real(np.diag(o))
+        w = np.exp(-decay * nph)
+        cols = {}  # qa -> list of (qs, qb, ncols)
+        for qb in sorted(right):
+            for qs in sorted(st.sec):
+                qa = qb - qs
+                if 0 <= qa <= Q:
+                    cols.setdefault(qa, []).append((qs, qb))
+        if i == 0:
+            cols = {0: cols[0]}
+            share = {0: 1}
+        else:
+            share = {}
+            per = max(1, D // len(cols))
+            for qa, lst in cols.items():
+                avail = sum(len(st.sec[qs]) * right[qb] for qs, qb in lst)
+                share[qa] = min(per, avail)
+        blocks, left = {}, {}
+        for qa, lst in cols.items():
+            widths = [len(st.sec[qs]) * right[qb] for qs, qb in lst]
+            C = sum(widths)
+            X = rng.standard_normal((share[qa], C)) + 1j * rng.standard_normal((share[qa], C))
+            pos = 0
+            for (qs, qb), wd in zip(lst, widths):
+                wcol = np.repeat(w[st.sec[qs]], right[qb])
+                X[:, pos:pos + wd] *= wcol[None, :]
+                pos += wd
+            q, _ = np.linalg.qr(X.conj().T)
+            rows = q.conj().T  # orthonormal rows
+            pos = 0
+            for (qs, qb), wd in zip(lst, widths):
+                blocks[(qa, qs, qb)] = rows[:, pos:pos + wd].reshape(share[qa], len(st.sec[qs]), right[qb])
+                pos += wd
+            left[qa] = share[qa]
+        out[i] = blocks
+        right = left
+    return out
