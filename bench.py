@@ -61,6 +61,21 @@ def run_path(kv, ch, psi0, precision, steps, warmup):
     return dict(wall=wall, ct=cts, stats=st, bonds=eng.bond_dims())
 
 
+def physical_gate(kv, ch, steps):
+    """max|C_mixed(t) - C_fp64(t)| from the physical start (exciton on site 0, phonon
+    vacuum), C(t) = <psi(0)|psi(t)>; not timed."""
+    cts = {}
+    for precision in ("fp64", "fp32"):
+        eng = kv.TDVP(ch.psi0, ch.mpo, cutoff=CUTOFF, maxdim=D, dt=DT, precision=precision)
+        cts[precision] = []
+        for k in range(steps):
+            eng.two_site()
+            cts[precision].append(kv.overlap_states(ch.psi0, eng.mps))
+        log(f"physical start {precision}: {steps} steps, max bond {max(eng.bond_dims())}")
+    c64, c32 = np.array(cts["fp64"]), np.array(cts["fp32"])
+    return float(np.max(np.abs(c32 - c64))), [[float(c.real), float(c.imag)] for c in c64]
+
+
 def cpu_baseline(kv, ch, psi0):
     """The oracle (numpy restatement of the reference algorithm) on one bond update of the
     same workload: 2-site Lanczos + SVD + environment update + 1-site backward Lanczos at
@@ -124,6 +139,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--physical-steps", type=int, default=6,
+                    help="steps of the physical-start run used for the mixed-vs-FP64 gate")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: independent chains would shard across GPUs with no "
@@ -140,6 +157,8 @@ def main():
     s64 = summarize(r64, "fp64", args.steps)
     s32 = summarize(r32, "fp32", args.steps)
     dC = float(np.max(np.abs(np.array(r32["ct"]) - np.array(r64["ct"]))))
+    dC_phys, ct_phys = (physical_gate(kv, ch, args.physical_steps) if args.physical_steps > 0
+                        else (None, None))
     cpu = None if args.no_cpu else cpu_baseline(kv, ch, psi0)
     ach = s32["heff_gemm_tflops"]
     out = {
@@ -166,6 +185,9 @@ def main():
         "speedup_vs_fp64": s32["steps_per_s"] / s64["steps_per_s"],
         "heff_speedup_vs_fp64": (s32["heff_tflops"] / s64["heff_tflops"]) if s64["heff_tflops"] else None,
         "max_dC_vs_fp64": dC,
+        "max_dC_vs_fp64_physical_start": dC_phys,
+        "physical_steps": args.physical_steps,
+        "C_t_fp64_physical_start": ct_phys,
         "roofline": {"kernel": "zgemm_grouped_kernel<float> (H_eff matvec stages)", "bound": "mfma",
                      "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": ach / PEAK["fp32"], "traffic": None,

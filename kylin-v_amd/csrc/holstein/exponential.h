@@ -7,7 +7,8 @@
 //   k < MaxIter-1 (:109, :156-159);  T = diag(alpha) + upper(beta), Hermitian
 //   eigen-decomposition, c = V exp(-/+ i Lambda) V^H e_0 (:162-183);  x = sum c_i v_i
 //   without renormalisation (:184-186).
-// The H_eff contractions may run in complex FP32 (prec); everything in this file is FP64.
+// The H_eff contractions may run in complex FP32 (prec) from the second Krylov vector on;
+// the first matvec and everything else in this file are FP64.
 #pragma once
 
 #include <chrono>
@@ -63,7 +64,10 @@ inline KrylovExit local_krylov(const std::shared_ptr<Device>& dev, int nsite, co
     double host[6];
     while (k < max_iter - 1 && err > tol) {
         double2* vk = Vb + (int64_t)k * n;
-        H.apply(vk, hv.as<double2>(), dt * 0.5, log);
+        // Mixed precision: the first matvec runs in FP64.  Its rounding error enters the
+        // result at first order in dt/2 |H v|; the errors of later Krylov vectors are
+        // damped by further powers of the Lanczos coefficients, so FP32 suffices there.
+        H.apply(vk, hv.as<double2>(), dt * 0.5, log, k == 0);
         if (stats) {
             stats->heff_flop += H.flop();
             stats->matvecs += 1;

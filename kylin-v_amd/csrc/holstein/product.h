@@ -191,7 +191,8 @@ struct Heff {
         if (nsite >= 1) s2.build(*dev, S.s2);
         if (nsite == 2) s3.build(*dev, S.s3);
         s4.build(*dev, S.s4);
-        const size_t es = (p == PREC_F64) ? sizeof(double2) : sizeof(float2);
+        // intermediates sized for FP64 so either precision can run through them
+        const size_t es = sizeof(double2);
         t1 = DevBuf(dev->pool.get(), (size_t)S.s1.out.total * es);
         if (nsite >= 1) t2 = DevBuf(dev->pool.get(), (size_t)S.s2.out.total * es);
         if (nsite == 2) t3 = DevBuf(dev->pool.get(), (size_t)S.s3.out.total * es);
@@ -221,9 +222,11 @@ struct Heff {
         }
         s4.run<T2>(*dev, last, Rd, y, alpha, 0.0, log);
     }
-    // y = alpha * H_eff x, x and y FP64 slabs in layout() order
-    void apply(const double2* x, double2* y, double alpha, EventLog* log = nullptr) {
-        if (prec == PREC_F64) {
+    // y = alpha * H_eff x, x and y FP64 slabs in layout() order; force64 runs the
+    // contractions in FP64 whatever the plan's precision
+    void apply(const double2* x, double2* y, double alpha, EventLog* log = nullptr,
+               bool force64 = false) {
+        if (prec == PREC_F64 || force64) {
             chain<double2>(L->d(), R->d(), x, y, alpha, log);
         } else {
             launch_f64_to_f32(dev->stream, x, x32.as<float2>(), S.xl.total);

@@ -22,6 +22,7 @@
 
 #include <cmath>
 #include <cstdlib>
+#include <map>
 #include <memory>
 #include <string>
 #include <vector>
@@ -138,8 +139,8 @@ inline void exact_sector(Device& D, Solver& sol, SectorSvd& X, bool keep_zero) {
     KV_REQUIRE(hinfo == 0, ERR_SOLVER, "Problems in SVD (did not converge)");
 }
 
-// Values from the Gram eigenproblem; G keeps the eigenvectors (row j = conj(v_j), ascending).
-inline void gram_values(Device& D, Solver& sol, SectorSvd& X) {
+// Gram matrix of a sector (G = M^H M when cols <= rows, else M M^H).
+inline void gram_prepare(Device& D, SectorSvd& X) {
     X.colside = X.nc <= X.nr;
     const int n = (int)(X.colside ? X.nc : X.nr);
     X.k = n;
@@ -150,24 +151,53 @@ inline void gram_values(Device& D, Solver& sol, SectorSvd& X) {
         gemm_dense(D, 'C', 'N', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nr);
     else  // G = M M^H
         gemm_dense(D, 'N', 'C', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nc);
-    DevBuf W(D.pool.get(), (size_t)n * sizeof(double)), E(D.pool.get(), (size_t)n * sizeof(double));
-    DevBuf info(D.pool.get(), sizeof(rocblas_int));
-    // the row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and
-    // column j of the returned vectors is conj(v_j), i.e. row j of the buffer read
-    // row-major is v_j^H
-    const rocblas_status st = rocsolver_zheevd(
-        sol.h, rocblas_evect_original, rocblas_fill_upper, n,
-        reinterpret_cast<rocblas_double_complex*>(X.G.get()), n, W.as<double>(), E.as<double>(),
-        info.as<rocblas_int>());
-    KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
-    std::vector<double> w(n);
-    rocblas_int hinfo = 0;
-    KV_HIP(hipMemcpyAsync(w.data(), W.get(), n * sizeof(double), hipMemcpyDeviceToHost, D.stream));
-    KV_HIP(hipMemcpyAsync(&hinfo, info.get(), sizeof(rocblas_int), hipMemcpyDeviceToHost, D.stream));
+}
+
+// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n, in one
+// rocSOLVER call (zheevd_strided_batched when there are several: its tridiagonalisation
+// is launch-latency bound, so batching sectors costs about as much as one).  The
+// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and
+// column j of the returned vectors is conj(v_j), i.e. row j of the buffer read row-major
+// is v_j^H (ascending eigenvalues).
+inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs) {
+    const int n = xs[0]->k;
+    const int nb = (int)xs.size();
+    DevBuf W(D.pool.get(), (size_t)n * nb * sizeof(double)), E(D.pool.get(), (size_t)n * nb * sizeof(double));
+    DevBuf info(D.pool.get(), sizeof(rocblas_int) * nb);
+    DevBuf Gb;
+    rocblas_double_complex* gp;
+    if (nb == 1) {
+        gp = reinterpret_cast<rocblas_double_complex*>(xs[0]->G.get());
+        const rocblas_status st = rocsolver_zheevd(sol.h, rocblas_evect_original, rocblas_fill_upper,
+                                                   n, gp, n, W.as<double>(), E.as<double>(),
+                                                   info.as<rocblas_int>());
+        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
+    } else {
+        const size_t nn = (size_t)n * n;
+        Gb = DevBuf(D.pool.get(), nn * nb * sizeof(double2));
+        for (int b = 0; b < nb; ++b)
+            KV_HIP(hipMemcpyAsync(Gb.as<double2>() + b * nn, xs[b]->G.get(), nn * sizeof(double2),
+                                  hipMemcpyDeviceToDevice, D.stream));
+        gp = reinterpret_cast<rocblas_double_complex*>(Gb.get());
+        const rocblas_status st = rocsolver_zheevd_strided_batched(
+            sol.h, rocblas_evect_original, rocblas_fill_upper, n, gp, n, (rocblas_stride)nn,
+            W.as<double>(), n, E.as<double>(), n, info.as<rocblas_int>(), nb);
+        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd_strided_batched failed");
+        for (int b = 0; b < nb; ++b)
+            KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
+                                  hipMemcpyDeviceToDevice, D.stream));
+    }
+    std::vector<double> w((size_t)n * nb);
+    std::vector<rocblas_int> hinfo(nb);
+    KV_HIP(hipMemcpyAsync(w.data(), W.get(), w.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    KV_HIP(hipMemcpyAsync(hinfo.data(), info.get(), nb * sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                          D.stream));
     D.sync();
-    KV_REQUIRE(hinfo == 0, ERR_SOLVER, "zheevd did not converge");
-    X.s.resize(n);
-    for (int j = 0; j < n; ++j) X.s[j] = std::sqrt(std::max(w[n - 1 - j], 0.0));
+    for (int b = 0; b < nb; ++b) {
+        KV_REQUIRE(hinfo[b] == 0, ERR_SOLVER, "zheevd did not converge");
+        xs[b]->s.resize(n);
+        for (int j = 0; j < n; ++j) xs[b]->s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0));
+    }
 }
 
 // Form the nk leading singular vectors of a Gram-route sector.
@@ -272,9 +302,16 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         const bool gram = !opt.force_exact && cutoff >= 0.0 &&
                           std::min(rs.len, cs.len) > opt.gram_min;
         if (gram)
-            gram_values(D, sol, X[p]);
+            gram_prepare(D, X[p]);
         else
             exact_sector(D, sol, X[p], cutoff < 0.0);
+    }
+    // Gram eigenproblems, batched by size
+    {
+        std::map<int, std::vector<SectorSvd*>> by_n;
+        for (int p = 0; p < np; ++p)
+            if (X[p].gram) by_n[X[p].k].push_back(&X[p]);
+        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second);
     }
     // phase 2: global truncation; Gram sectors that would keep values too small for the
     // Gram eigenvalues' absolute accuracy move to the exact route and the rule reruns

@@ -223,8 +223,11 @@ class TDVP {
     bool timing = true;
 
   private:
+    // Environments stay FP64 in every precision mode: they enter every later matvec, so
+    // their rounding error would bias the effective Hamiltonian itself (they cost ~5% of
+    // the H_eff work).
     std::unique_ptr<Tensor> update(const Tensor& env, const Tensor& A, Tensor& W, bool l2r) {
-        auto out = env_update(dev_, env, nullptr, W, A, l2r, cfg_.prec, timing ? &log_ : nullptr,
+        auto out = env_update(dev_, env, nullptr, W, A, l2r, PREC_F64, timing ? &log_ : nullptr,
                               &env_flop_);
         if (timing) log_.resolve(env_k_);
         return out;
