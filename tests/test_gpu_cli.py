@@ -1,0 +1,69 @@
+"""kln-tdvp command-line driver (src/kln-tdvp.cpp contract) on the GPU: reference-format
+files in, State/T-<k>.mps out, C(t) against the oracle."""
+import os
+import struct
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import model as M
+from oracle import tdvp as T
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+sys.path.insert(0, ROOT)
+import __graft_entry__ as GE  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+def write_train(path, tensors, magic, rank):
+    with open(path, "wb") as f:
+        f.write(struct.pack("ii", magic, len(tensors)))
+        for t in tensors:
+            items = t.sorted_items()
+            f.write(struct.pack("i", len(items)))
+            for q, b in items:
+                f.write(struct.pack(f"{rank}i", *b.shape))
+                f.write(struct.pack(f"{rank}i", *q))
+                f.write(np.ascontiguousarray(b).tobytes())
+        f.write(b"\n")
+
+
+@pytest.mark.parametrize("layout", ["merged", "pp"])
+def test_kln_tdvp_cli_matches_oracle(tmp_path, layout):
+    print(f"case kln-tdvp {layout}", flush=True)
+    kv = GE.load_package()
+    g = np.array([1.0, 0.8, 1.2])
+    lat = M.holstein(3, [1.0], g, [3], 1.0, layout)
+    mpo = M.build_mpo(lat)
+    psi0 = M.product_state(lat)
+    write_train(tmp_path / "Ele-1.mps", psi0, 111, 3)
+    write_train(tmp_path / "H_tot", mpo, 112, 4)
+    exe = os.path.join(ROOT, "kylin-v_amd", "kln-tdvp")
+    r = subprocess.run([exe, "-w", "Ele-1.mps", "-o", "H_tot", "-s", "[0,3,0,1]", "-v", "0.05",
+                        "-D", "12", "-t", "1e-10"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    print(r.stdout[-800:], r.stderr[-800:], flush=True)
+    assert r.returncode == 0
+    assert "Max bond dimension" in r.stdout
+    ref = T.TDVP(psi0, mpo, cutoff=1e-10, maxdim=12, dt=0.05)
+    for k in range(4):
+        if k < 3:
+            ref.two_site()
+        else:
+            ref.one_site()
+        sites = kv.load_mps(str(tmp_path / "State" / f"T-{k}.mps"))
+        o_sites = []
+        from oracle.blocksparse import Tensor as OT
+        for s in sites:
+            dirs, blocks = s.to_blocks()
+            o_sites.append(OT(dirs, blocks))
+        c_cli = T.overlap_states(psi0, o_sites)
+        c_ref = T.overlap_states(psi0, ref.mps)
+        assert abs(c_cli - c_ref) < 1e-9
+    # bad stage: the global-Krylov stage is rejected with a message, not a crash
+    r2 = subprocess.run([exe, "-w", "Ele-1.mps", "-o", "H_tot", "-s", "[1,3,0,0]", "-v", "0.05"],
+                        cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r2.returncode == 1 and "not supported" in r2.stderr
