@@ -23,6 +23,18 @@ inline int op_code(char op) {
     throw Error(ERR_ARG, std::string("bad op code ") + op);
 }
 
+// Output tiles of a group list for tile config cfg (kGemmConfigs).
+inline std::vector<GemmTile> cut_tiles(const std::vector<GemmGroup>& groups, int cfg) {
+    const int bm = kGemmConfigs[cfg].bm, bn = kGemmConfigs[cfg].bn;
+    std::vector<GemmTile> t;
+    for (int gi = 0; gi < (int)groups.size(); ++gi) {
+        const int tm = (groups[gi].m + bm - 1) / bm, tn = (groups[gi].n + bn - 1) / bn;
+        for (int a = 0; a < tm; ++a)
+            for (int b = 0; b < tn; ++b) t.push_back(GemmTile{gi, a, b, 0});
+    }
+    return t;
+}
+
 struct ProductPlan {
     int opA = 0, opB = 0;
     Layout out;
@@ -178,12 +190,7 @@ inline ProductPlan plan_product(const Layout& A, const Layout& B, int nc, char o
         g.seg_count = 0;
         P.groups.push_back(g);
     }
-    for (int gi = 0; gi < (int)P.groups.size(); ++gi) {
-        const int tm = (P.groups[gi].m + GEMM_BM - 1) / GEMM_BM;
-        const int tn = (P.groups[gi].n + GEMM_BN - 1) / GEMM_BN;
-        for (int a = 0; a < tm; ++a)
-            for (int b = 0; b < tn; ++b) P.tiles.push_back(GemmTile{gi, a, b, 0});
-    }
+    P.tiles = cut_tiles(P.groups, 0);
     P.out = std::move(out);
     return P;
 }

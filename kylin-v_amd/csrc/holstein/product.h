@@ -60,26 +60,95 @@ struct EventLog {
     }
 };
 
+// Shape class of a product plan for the tile autotuner: log2 buckets of the m, n and total
+// k of its largest group.
+inline uint64_t gemm_shape_key(const ProductPlan& p, int prec) {
+    int64_t best = -1;
+    int bm = 0, bn = 0, bk = 0;
+    for (const auto& g : p.groups) {
+        int64_t k = 0;
+        for (int s = 0; s < g.seg_count; ++s) k += p.segs[g.seg_begin + s].k;
+        const int64_t w = (int64_t)g.m * g.n * std::max<int64_t>(k, 1);
+        if (w > best) {
+            best = w;
+            bm = g.m;
+            bn = g.n;
+            bk = (int)std::min<int64_t>(k, 1 << 30);
+        }
+    }
+    auto lg = [](int v) {
+        int l = 0;
+        while ((1 << (l + 1)) <= v && l < 20) ++l;
+        return l;
+    };
+    return ((uint64_t)prec << 40) | ((uint64_t)p.opA << 36) | ((uint64_t)p.opB << 32) |
+           ((uint64_t)lg(bm) << 20) | ((uint64_t)lg(bn) << 10) | (uint64_t)lg(bk);
+}
+
 struct DevProduct {
     ProductPlan p;
-    DevBuf tiles, groups, segs;
+    DevBuf groups, segs;
+    DevBuf tiles[kNumGemmConfigs];
+    int ntiles[kNumGemmConfigs] = {0};
+    bool cut[kNumGemmConfigs] = {false};
     void build(Device& dev, ProductPlan plan) {
         p = std::move(plan);
-        tiles = upload_vec(dev, p.tiles);
         groups = upload_vec(dev, p.groups);
         segs = upload_vec(dev, p.segs);
+        tiles[0] = upload_vec(dev, p.tiles);
+        ntiles[0] = (int)p.tiles.size();
+        cut[0] = true;
+    }
+    void ensure_tiles(Device& dev, int cfg) {
+        if (cut[cfg]) return;
+        std::vector<GemmTile> t = cut_tiles(p.groups, cfg);
+        tiles[cfg] = upload_vec(dev, t);
+        ntiles[cfg] = (int)t.size();
+        cut[cfg] = true;
+    }
+    template <class T2>
+    void launch(Device& dev, int cfg, const T2* A, const T2* B, T2* C, double ar, double ai) {
+        ensure_tiles(dev, cfg);
+        launch_gemm<T2>(dev.stream, p.opA, p.opB, A, B, C, tiles[cfg].as<GemmTile>(), ntiles[cfg],
+                        groups.as<GemmGroup>(), segs.as<GemmSeg>(), ar, ai, cfg);
+    }
+    // Tile config for this plan: cached per shape class; on a first encounter every config
+    // built for the op pair runs twice on the real operands and the fastest is kept.
+    template <class T2>
+    int choose(Device& dev, const T2* A, const T2* B, T2* C, double ar, double ai) {
+        if (!dev.autotune || p.flop < 1e8) return 0;
+        const uint64_t key = gemm_shape_key(p, sizeof(T2) == sizeof(double2) ? 0 : 1);
+        auto it = dev.gemm_tuned.find(key);
+        if (it != dev.gemm_tuned.end()) return it->second;
+        EventTimer ev;
+        int best = 0;
+        float best_ms = 1e30f;
+        for (int cfg = 0; cfg < kNumGemmConfigs; ++cfg) {
+            if (!gemm_config_supports(cfg, p.opA, p.opB)) continue;
+            launch<T2>(dev, cfg, A, B, C, ar, ai);
+            ev.start(dev.stream);
+            launch<T2>(dev, cfg, A, B, C, ar, ai);
+            ev.stop(dev.stream);
+            const float ms = ev.ms();
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cfg;
+            }
+        }
+        dev.gemm_tuned[key] = best;
+        return best;
     }
     template <class T2>
     void run(Device& dev, const T2* A, const T2* B, T2* C, double alpha_re = 1.0,
-             double alpha_im = 0.0, EventLog* log = nullptr) const {
+             double alpha_im = 0.0, EventLog* log = nullptr) {
+        const int cfg = choose<T2>(dev, A, B, C, alpha_re, alpha_im);
         hipEvent_t a = nullptr, b = nullptr;
         if (log) {
             a = log->get();
             b = log->get();
             KV_HIP(hipEventRecord(a, dev.stream));
         }
-        launch_gemm<T2>(dev.stream, p.opA, p.opB, A, B, C, tiles.as<GemmTile>(), (int)p.tiles.size(),
-                        groups.as<GemmGroup>(), segs.as<GemmSeg>(), alpha_re, alpha_im);
+        launch<T2>(dev, cfg, A, B, C, alpha_re, alpha_im);
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
             log->pending.push_back({a, b, p.flop, 0});

@@ -9,10 +9,12 @@
 
 namespace kv {
 
+// cfg selects the tile shape (kGemmConfigs in kv_types.h); tiles must have been cut for it
 template <class T2>
 void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2* C,
                  const GemmTile* tiles, int ntiles, const GemmGroup* groups,
-                 const GemmSeg* segs, double alpha_re, double alpha_im);
+                 const GemmSeg* segs, double alpha_re, double alpha_im, int cfg = 0);
+bool gemm_config_supports(int cfg, int opA, int opB);
 
 template <class T2>
 void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,

@@ -46,7 +46,11 @@ template <> struct ScalarOf<float2> { using T = float; };
 // ------------------------------------------------------------------------------------
 // grouped complex GEMM
 // ------------------------------------------------------------------------------------
-template <class T, int OPA, int OPB>
+// Tile BM x BN complex outputs per 256-thread workgroup; 4 waves in a 2x2 arrangement, each
+// owning (BM/2) x (BN/2) outputs as 16x16 MFMA sub-tiles; BK complex K per LDS stage.
+// Workgroups are dispatched round-robin over the 8 XCDs, so the grid is remapped to give
+// each XCD a contiguous run of tiles (tiles of one group row share A panels in that XCD's L2).
+template <class T, int OPA, int OPB, int BM, int BN, int BK>
 __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
     const typename Cx<T>::T2* __restrict__ A, const typename Cx<T>::T2* __restrict__ B,
     typename Cx<T>::T2* __restrict__ C, const GemmTile* __restrict__ tiles,
@@ -54,23 +58,27 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
     T alpha_im) {
     using T2 = typename Cx<T>::T2;
     using acc_t = typename Cx<T>::acc;
-    constexpr int BM = GEMM_BM, BN = GEMM_BN, BK = GEMM_BK;
+    constexpr int SI = BM / 32, SJ = BN / 32;  // 16x16 sub-tiles per wave
     __shared__ T2 As[BK][BM + 1];
     __shared__ T2 Bs[BK][BN + 1];
 
-    const GemmTile tile = tiles[blockIdx.x];
+    const int nt = gridDim.x;
+    const int x = blockIdx.x & 7, i8 = blockIdx.x >> 3;
+    const int per = nt >> 3, rem = nt & 7;
+    const int tidx = x * per + (x < rem ? x : rem) + i8;
+    const GemmTile tile = tiles[tidx];
     const GemmGroup g = groups[tile.group];
     const int m0 = tile.tm * BM, n0 = tile.tn * BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
 
-    acc_t cre[2][2], cim[2][2];
+    acc_t cre[SI][SJ], cim[SI][SJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
+        for (int j = 0; j < SJ; ++j) {
             cre[i][j] = acc_t{0, 0, 0, 0};
             cim[i][j] = acc_t{0, 0, 0, 0};
         }
@@ -81,7 +89,6 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
         const T2* __restrict__ b = B + sg.b_off;
         const int K = sg.k;
         for (int k0 = 0; k0 < K; k0 += BK) {
-            // stage A (BM x BK) as As[k][m]
 #pragma unroll
             for (int r = 0; r < (BM * BK) / 256; ++r) {
                 const int e = tid + r * 256;
@@ -103,7 +110,6 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
                 }
                 As[kk][mm] = v;
             }
-            // stage B (BK x BN) as Bs[k][n]
 #pragma unroll
             for (int r = 0; r < (BN * BK) / 256; ++r) {
                 const int e = tid + r * 256;
@@ -128,15 +134,15 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
             __syncthreads();
 #pragma unroll
             for (int kq = 0; kq < BK; kq += 4) {
-                T2 av[2], bv[2];
+                T2 av[SI], bv[SJ];
 #pragma unroll
-                for (int i = 0; i < 2; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
+                for (int i = 0; i < SI; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
 #pragma unroll
-                for (int j = 0; j < 2; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
+                for (int j = 0; j < SJ; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int i = 0; i < SI; ++i)
 #pragma unroll
-                    for (int j = 0; j < 2; ++j) {
+                    for (int j = 0; j < SJ; ++j) {
                         cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
                         cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
                         cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
@@ -148,9 +154,9 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
     }
     // epilogue: MFMA output layout per type (Cx<T>::out_row)
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < SJ; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm + i * 16 + Cx<T>::out_row(lk, v);
@@ -168,37 +174,61 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
 template <class T2>
 void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2* C,
                  const GemmTile* tiles, int ntiles, const GemmGroup* groups,
-                 const GemmSeg* segs, double alpha_re, double alpha_im) {
+                 const GemmSeg* segs, double alpha_re, double alpha_im, int cfg) {
     using T = typename ScalarOf<T2>::T;
     if (ntiles <= 0) return;
     const dim3 grid(ntiles), block(256);
     const T ar = (T)alpha_re, ai = (T)alpha_im;
-#define KV_GEMM_CASE(OA, OB)                                                              \
-    if (opA == OA && opB == OB) {                                                        \
-        hipLaunchKernelGGL((zgemm_grouped_kernel<T, OA, OB>), grid, block, 0, st, A, B, C, \
-                           tiles, groups, segs, ar, ai);                                   \
-        KV_HIP(hipGetLastError());                                                        \
-        return;                                                                           \
+#define KV_GEMM_CASE(C_, OA, OB, BM_, BN_, BK_)                                                  \
+    if (cfg == C_ && opA == OA && opB == OB) {                                                 \
+        hipLaunchKernelGGL((zgemm_grouped_kernel<T, OA, OB, BM_, BN_, BK_>), grid, block, 0, st, \
+                           A, B, C, tiles, groups, segs, ar, ai);                              \
+        KV_HIP(hipGetLastError());                                                             \
+        return;                                                                                \
     }
-    KV_GEMM_CASE(0, 0)
-    KV_GEMM_CASE(0, 1)
-    KV_GEMM_CASE(0, 2)
-    KV_GEMM_CASE(1, 0)
-    KV_GEMM_CASE(1, 1)
-    KV_GEMM_CASE(1, 2)
-    KV_GEMM_CASE(2, 0)
-    KV_GEMM_CASE(2, 1)
-    KV_GEMM_CASE(2, 2)
+#define KV_GEMM_ALL_OPS(C_, BM_, BN_, BK_)  \
+    KV_GEMM_CASE(C_, 0, 0, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 0, 1, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 0, 2, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 1, 0, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 1, 1, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 1, 2, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 2, 0, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 2, 1, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 2, 2, BM_, BN_, BK_)
+#define KV_GEMM_HOT_OPS(C_, BM_, BN_, BK_)  \
+    KV_GEMM_CASE(C_, 0, 0, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 0, 1, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 2, 0, BM_, BN_, BK_)   \
+    KV_GEMM_CASE(C_, 0, 2, BM_, BN_, BK_)
+    KV_GEMM_ALL_OPS(0, 64, 64, 8)
+    KV_GEMM_HOT_OPS(1, 64, 64, 16)
+    KV_GEMM_HOT_OPS(2, 128, 64, 8)
+    KV_GEMM_HOT_OPS(3, 64, 128, 8)
+    KV_GEMM_HOT_OPS(4, 128, 128, 8)
+    KV_GEMM_HOT_OPS(5, 128, 64, 16)
+    KV_GEMM_HOT_OPS(6, 64, 128, 16)
+#undef KV_GEMM_HOT_OPS
+#undef KV_GEMM_ALL_OPS
 #undef KV_GEMM_CASE
+    if (cfg != 0) {  // op pair without a specialised tile: fall back to the 64x64 tile
+        throw Error(ERR_INTERNAL, "launch_gemm: tile config not built for this op pair");
+    }
     throw Error(ERR_ARG, "launch_gemm: bad op codes");
+}
+
+bool gemm_config_supports(int cfg, int opA, int opB) {
+    if (cfg == 0) return opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2;
+    return (opA == 0 && opB == 0) || (opA == 0 && opB == 1) || (opA == 2 && opB == 0) ||
+           (opA == 0 && opB == 2);
 }
 
 template void launch_gemm<double2>(hipStream_t, int, int, const double2*, const double2*,
                                    double2*, const GemmTile*, int, const GemmGroup*,
-                                   const GemmSeg*, double, double);
+                                   const GemmSeg*, double, double, int);
 template void launch_gemm<float2>(hipStream_t, int, int, const float2*, const float2*,
                                   float2*, const GemmTile*, int, const GemmGroup*,
-                                  const GemmSeg*, double, double);
+                                  const GemmSeg*, double, double, int);
 
 // ------------------------------------------------------------------------------------
 // MPO fold

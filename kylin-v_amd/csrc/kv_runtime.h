@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -137,10 +138,14 @@ struct Device {
     std::unique_ptr<Pool> pool;
     bool timing = false;  // record HIP events around contraction kernels
     KernelStats stats;
+    // GEMM tile autotuner: (precision, opA, opB, shape class) -> tile config
+    bool autotune = true;
+    std::map<uint64_t, int> gemm_tuned;
     explicit Device(int dev) : id(dev) {
         KV_HIP(hipSetDevice(dev));
         KV_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         pool.reset(new Pool(dev));
+        if (const char* a = std::getenv("KV_AUTOTUNE")) autotune = a[0] != '0';
     }
     ~Device() {
         if (stream) {

@@ -25,6 +25,16 @@ struct GemmTile {
 
 constexpr int GEMM_BM = 64, GEMM_BN = 64, GEMM_BK = 8;
 
+// Tile shapes of the grouped GEMM (complex outputs per workgroup, K depth per LDS stage).
+// Config 0 exists for every op pair; the others for the pairs on the hot path
+// (NN, NT, CN, NC).
+struct GemmConfig {
+    int bm, bn, bk;
+};
+constexpr int kNumGemmConfigs = 7;
+constexpr GemmConfig kGemmConfigs[kNumGemmConfigs] = {
+    {64, 64, 8}, {64, 64, 16}, {128, 64, 8}, {64, 128, 8}, {128, 128, 8}, {128, 64, 16}, {64, 128, 16}};
+
 // ---- MPO fold -----------------------------------------------------------------------
 // out[out_base + p*out_pstride + q] = sum_t w_t * in[t.in_base + p*t.in_pstride + q]
 // for p < P, q < Q.  A task with no terms writes zeros.

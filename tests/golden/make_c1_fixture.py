@@ -1,0 +1,48 @@
+"""Writes tests/golden/c1_<layout>.json: the oracle's C(t), exciton populations and bond
+dimensions for the 8-site Holstein chain (exciton x one mode at 8 levels, J = w = g = 1,
+two-site TDVP dt = 0.05, 40 steps, D <= 32, cutoff 1e-10, Krylov tol 1e-8, 15 iterations).
+
+The oracle (oracle/) is the numpy restatement of the reference's algorithm, pinned by
+tests/test_oracle.py; these files let the GPU test compare the whole 40-step run without
+re-running the oracle on the GPU box.
+
+    python tests/golden/make_c1_fixture.py [merged|pp ...]
+"""
+import json
+import os
+import sys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from oracle import model as M  # noqa: E402
+from oracle import tdvp as T  # noqa: E402
+
+
+def run(layout, steps=40):
+    p = M.c1_params()
+    lat = M.holstein(p["N"], p["omegas"], p["gs"], p["Ls"], p["J"], layout)
+    mpo = M.build_mpo(lat)
+    psi0 = M.product_state(lat)
+    eng = T.TDVP(psi0, mpo, cutoff=1e-10, maxdim=32, dt=0.05, tol_krylov=1e-8, max_iter=15)
+    out = {"layout": layout, "N": p["N"], "L": p["Ls"][0], "J": p["J"], "omega": p["omegas"][0],
+           "g": 1.0, "dt": 0.05, "D": 32, "cutoff": 1e-10, "steps": steps,
+           "C": [], "pop_steps": [], "pops": [], "bonds": []}
+    dens = [M.density_mpo(lat, i) for i in range(p["N"])]
+    for k in range(1, steps + 1):
+        eng.two_site()
+        c = T.overlap_states(psi0, eng.mps)
+        out["C"].append([c.real, c.imag])
+        if k % 10 == 0:
+            out["pop_steps"].append(k)
+            out["pops"].append([T.join(d, eng.mps, eng.mps).real for d in dens])
+            out["bonds"].append(eng.bond_dims())
+        print(layout, k, c, flush=True)
+    return out
+
+
+if __name__ == "__main__":
+    for layout in (sys.argv[1:] or ["merged", "pp"]):
+        d = run(layout)
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), f"c1_{layout}.json"), "w") as f:
+            json.dump(d, f)
