@@ -23,7 +23,7 @@ inline int op_code(char op) {
     throw Error(ERR_ARG, std::string("bad op code ") + op);
 }
 
-// Output tiles of a group list for tile config cfg (kGemmConfigs).
+// Output tiles of a group list for tile config cfg (kGemmConfigs), row-major per group.
 inline std::vector<GemmTile> cut_tiles(const std::vector<GemmGroup>& groups, int cfg) {
     const int bm = kGemmConfigs[cfg].bm, bn = kGemmConfigs[cfg].bn;
     std::vector<GemmTile> t;

@@ -2,6 +2,8 @@
 // effective-Hamiltonian matvec and the environment update, all on the context stream.
 #pragma once
 
+#include <cstdio>
+#include <cstdlib>
 #include <memory>
 #include <vector>
 
@@ -123,6 +125,12 @@ struct DevProduct {
         EventTimer ev;
         int best = 0;
         float best_ms = 1e30f;
+        const char* vb = std::getenv("KV_VERBOSE");
+        const bool verbose = vb && vb[0] == '1';
+        char line[512];
+        int len = std::snprintf(line, sizeof line, "[kv] autotune %s op%d%d %.3g GFLOP:",
+                                sizeof(T2) == sizeof(double2) ? "f64" : "f32", p.opA, p.opB,
+                                p.flop * 1e-9);
         for (int cfg = 0; cfg < kNumGemmConfigs; ++cfg) {
             if (!gemm_config_supports(cfg, p.opA, p.opB)) continue;
             launch<T2>(dev, cfg, A, B, C, ar, ai);
@@ -130,11 +138,16 @@ struct DevProduct {
             launch<T2>(dev, cfg, A, B, C, ar, ai);
             ev.stop(dev.stream);
             const float ms = ev.ms();
+            if (verbose && len < (int)sizeof line - 40)
+                len += std::snprintf(line + len, sizeof line - len, " %dx%dx%d=%.3fms",
+                                     kGemmConfigs[cfg].bm, kGemmConfigs[cfg].bn,
+                                     kGemmConfigs[cfg].bk, ms);
             if (ms < best_ms) {
                 best_ms = ms;
                 best = cfg;
             }
         }
+        if (verbose) std::fprintf(stderr, "%s -> %d\n", line, best);
         dev.gemm_tuned[key] = best;
         return best;
     }
