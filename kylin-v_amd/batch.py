@@ -28,10 +28,16 @@ def c3_chains(n=64, seed=3):
     return out
 
 
+MAX_WORKERS = 8
+
+
 def run_batch(specs, steps, dt=0.05, cutoff=1e-10, workers=4, precision="fp64",
               layout="merged", on_progress=None):
     """Advance every chain `steps` two-site TDVP steps.  Returns (results, wall seconds)
-    where results[i] = dict(C=[...], bonds=[...], seconds=...) in input order."""
+    where results[i] = dict(C=[...], bonds=[...], seconds=...) in input order.  At most
+    MAX_WORKERS contexts run at once (16 concurrent contexts exhausted HSA queue resources
+    on the MI355X box)."""
+    workers = max(1, min(workers, MAX_WORKERS))
     from . import Context, TDVP, overlap_states
     from . import model as KM
 
@@ -42,7 +48,8 @@ def run_batch(specs, steps, dt=0.05, cutoff=1e-10, workers=4, precision="fp64",
 
     def worker(wid):
         try:
-            ctx = Context(0)
+            with lock:  # contexts are created one at a time
+                ctx = Context(0)
             while True:
                 with lock:
                     i = next_idx[0]

@@ -24,6 +24,7 @@
 #include <cstdlib>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -32,9 +33,18 @@
 
 namespace kv {
 
+// rocBLAS initialises its Tensile host state lazily on the first handle; concurrent first
+// calls from several threads fail ("Could not initialize Tensile host"), so handle
+// creation is serialised process-wide.
+inline std::mutex& solver_init_mutex() {
+    static std::mutex m;
+    return m;
+}
+
 struct Solver {
     rocblas_handle h = nullptr;
     explicit Solver(Device& dev) {
+        std::lock_guard<std::mutex> g(solver_init_mutex());
         if (rocblas_create_handle(&h) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_create_handle failed");
         if (rocblas_set_stream(h, dev.stream) != rocblas_status_success)
