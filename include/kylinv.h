@@ -35,7 +35,10 @@ typedef struct kv_tdvp kv_tdvp;
  * scalars, vector algebra, the Krylov exponential and the bond SVD are FP64 always. */
 enum kv_precision {
     KV_FP64 = 0,   /* complex FP64 on v_mfma_f64_16x16x4f64                        */
-    KV_FP32 = 1    /* complex FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation     */
+    KV_FP32 = 1,   /* complex FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation     */
+    KV_BF16X3 = 2  /* FP32 storage; H_eff GEMMs as split-BF16 products (hi*hi + hi*lo +
+                      lo*hi) on v_mfma_f32_16x16x32_bf16, FP32 accumulation; environment
+                      updates requested in this mode run their GEMMs in FP32              */
 };
 
 /* Status codes. */
@@ -61,6 +64,10 @@ int kv_ctx_destroy(kv_ctx* ctx);
 const char* kv_last_error(const kv_ctx* ctx);
 /* Block until all work queued on the context's stream has finished. */
 int kv_sync(kv_ctx* ctx);
+/* Return the context's cached device buffers to the device (live tensors are kept).
+ * The reference has no counterpart (it mallocs per block); for long-lived contexts that
+ * run many differently shaped problems. */
+int kv_ctx_trim(kv_ctx* ctx);
 
 /* ---- block-sparse tensors (holstein/block.h:23-162, holstein/tensor.h:16-212) ---- */
 /* Create a tensor with the given block structure, zero-filled.  qn and shape are
@@ -88,6 +95,16 @@ int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im);
  * legs of m1 / last nc legs of m2; 'C' conjugates. */
 int kv_product(kv_ctx* ctx, const kv_tensor* m1, const kv_tensor* m2, int nc,
                char op1, char op2, int precision, kv_tensor** out);
+
+/* Tensor algebra used by the TDVP adjoint.
+ * kv_tensor_transpose: transpose(t, axes) (holstein/tensor.h:243-260), out leg a = t leg axes[a].
+ * kv_tensor_axpy: out = y + alpha x over the union of both block sets (operator+=,
+ *   holstein/tensor.h:88-105, which axpys into equal blocks and appends new ones).
+ * kv_tensor_scale: t *= alpha in place (operator* / operator*=, holstein/tensor.h:107-123). */
+int kv_tensor_transpose(kv_ctx* ctx, const kv_tensor* t, const int* axes, kv_tensor** out);
+int kv_tensor_axpy(kv_ctx* ctx, const kv_tensor* x, const kv_tensor* y, double alpha_re,
+                   double alpha_im, kv_tensor** out);
+int kv_tensor_scale(kv_ctx* ctx, kv_tensor* t, double alpha_re, double alpha_im);
 
 /* Effective Hamiltonian y = L . W1 . W2 . x . R.  nsite = 2 replaces apply_double
  * (holstein/product.h:478-486, W1 and W2 used); nsite = 1 replaces apply_single

@@ -28,8 +28,9 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libkylinv.so")
 
-KV_FP64, KV_FP32 = 0, 1
-_PREC = {"fp64": KV_FP64, "fp32": KV_FP32, KV_FP64: KV_FP64, KV_FP32: KV_FP32}
+KV_FP64, KV_FP32, KV_BF16X3 = 0, 1, 2
+_PREC = {"fp64": KV_FP64, "fp32": KV_FP32, "bf16x3": KV_BF16X3,
+         KV_FP64: KV_FP64, KV_FP32: KV_FP32, KV_BF16X3: KV_BF16X3}
 
 
 class KVError(RuntimeError):
@@ -63,6 +64,7 @@ SIGNATURES = {
     "kv_ctx_destroy": (_I, [_P]),
     "kv_last_error": (ctypes.c_char_p, [_P]),
     "kv_sync": (_I, [_P]),
+    "kv_ctx_trim": (_I, [_P]),
     "kv_tensor_create": (_I, [_P, _I, _IP, _I, _IP, _IP, _PP]),
     "kv_tensor_destroy": (_I, [_P]),
     "kv_tensor_rank": (_I, [_P, _IP, _IP, _LP]),
@@ -73,6 +75,9 @@ SIGNATURES = {
     "kv_tensor_norm": (_I, [_P, _DP]),
     "kv_tensor_overlap": (_I, [_P, _P, _DP]),
     "kv_product": (_I, [_P, _P, _P, _I, ctypes.c_char, ctypes.c_char, _I, _PP]),
+    "kv_tensor_transpose": (_I, [_P, _P, _IP, _PP]),
+    "kv_tensor_axpy": (_I, [_P, _P, _P, _D, _D, _PP]),
+    "kv_tensor_scale": (_I, [_P, _P, _D, _D]),
     "kv_heff_apply": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _PP]),
     "kv_env_update": (_I, [_P, _P, _P, _P, _P, _I, _I, _PP]),
     "kv_krylov_step": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _I, _D, _I, _I,
@@ -131,6 +136,10 @@ class Context:
 
     def sync(self):
         self.check(self.lib.kv_sync(self.h))
+
+    def trim(self):
+        """Release cached device buffers (kv_ctx_trim)."""
+        self.check(self.lib.kv_ctx_trim(self.h))
 
     def close(self):
         if getattr(self, "h", None):
@@ -258,6 +267,33 @@ def product(m1: Tensor, m2: Tensor, nc: int, t1: str = "N", t2: str = "N") -> Te
     ctx.check(ctx.lib.kv_product(ctx.h, m1.h, m2.h, nc, t1.encode(), t2.encode(), KV_FP64,
                                  ctypes.byref(h)))
     return Tensor(ctx, h)
+
+
+def transpose(t: Tensor, axes) -> Tensor:
+    """transpose(t, axes) (holstein/tensor.h:243-260): out leg a is t leg axes[a]."""
+    ctx = t.ctx
+    h = ctypes.c_void_p()
+    keep, ax = _iarr(list(axes))
+    ctx.check(ctx.lib.kv_tensor_transpose(ctx.h, t.h, ax, ctypes.byref(h)))
+    del keep
+    return Tensor(ctx, h)
+
+
+def axpy(x: Tensor, y: Tensor, alpha=1.0) -> Tensor:
+    """New tensor y + alpha x over the union of both block sets (operator+=,
+    holstein/tensor.h:88-105)."""
+    ctx = y.ctx
+    a = complex(alpha)
+    h = ctypes.c_void_p()
+    ctx.check(ctx.lib.kv_tensor_axpy(ctx.h, x.h, y.h, a.real, a.imag, ctypes.byref(h)))
+    return Tensor(ctx, h)
+
+
+def scale_(t: Tensor, alpha) -> Tensor:
+    """t *= alpha in place (holstein/tensor.h:107-123); returns t."""
+    a = complex(alpha)
+    t.ctx.check(t.ctx.lib.kv_tensor_scale(t.ctx.h, t.h, a.real, a.imag))
+    return t
 
 
 def _heff(nsite, L, W1, W2, R, x, precision):

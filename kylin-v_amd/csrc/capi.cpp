@@ -53,8 +53,8 @@ kv_tensor* wrap(kv_ctx* ctx, std::unique_ptr<kv::Tensor> t) {
 }
 
 kv::Prec prec_of(int p) {
-    KV_REQUIRE(p == KV_FP64 || p == KV_FP32, kv::ERR_ARG, "unknown precision");
-    return p == KV_FP64 ? kv::PREC_F64 : kv::PREC_F32;
+    KV_REQUIRE(p == KV_FP64 || p == KV_FP32 || p == KV_BF16X3, kv::ERR_ARG, "unknown precision");
+    return p == KV_FP64 ? kv::PREC_F64 : p == KV_FP32 ? kv::PREC_F32 : kv::PREC_BF16X3;
 }
 
 kv::Tensor* need(const kv_tensor* t, const char* what) {
@@ -90,6 +90,14 @@ int kv_ctx_destroy(kv_ctx* ctx) {
 }
 
 const char* kv_last_error(const kv_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int kv_ctx_trim(kv_ctx* ctx) {
+    if (!ctx) return KV_ERR_ARG;
+    return guard(ctx, [&] {
+        ctx->dev->sync();
+        ctx->dev->pool->trim(0);
+    });
+}
 
 int kv_sync(kv_ctx* ctx) {
     if (!ctx) return KV_ERR_ARG;
@@ -213,6 +221,36 @@ int kv_product(kv_ctx* ctx, const kv_tensor* m1, const kv_tensor* m2, int nc, ch
         auto C = kv::product(ctx->dev, *need(m1, "m1"), *need(m2, "m2"), nc, op1, op2);
         ctx->dev->sync();
         *out = wrap(ctx, std::move(C));
+    });
+}
+
+int kv_tensor_transpose(kv_ctx* ctx, const kv_tensor* t, const int* axes, kv_tensor** out) {
+    if (!ctx || !axes || !out) return KV_ERR_ARG;
+    return guard(ctx, [&] {
+        const kv::Tensor* T = need(t, "t");
+        kv::Idx ax{};
+        for (int a = 0; a < T->lay.rank; ++a) ax[a] = axes[a];
+        *out = wrap(ctx, kv::transpose(ctx->dev, *T, ax));
+    });
+}
+
+int kv_tensor_axpy(kv_ctx* ctx, const kv_tensor* x, const kv_tensor* y, double alpha_re,
+                   double alpha_im, kv_tensor** out) {
+    if (!ctx || !out) return KV_ERR_ARG;
+    return guard(ctx, [&] {
+        *out = wrap(ctx, kv::axpy(ctx->dev, *need(x, "x"), *need(y, "y"),
+                                  make_double2(alpha_re, alpha_im)));
+    });
+}
+
+int kv_tensor_scale(kv_ctx* ctx, kv_tensor* t, double alpha_re, double alpha_im) {
+    if (!ctx || !t) return KV_ERR_ARG;
+    return guard(ctx, [&] {
+        kv::Tensor* T = t->t.get();
+        KV_REQUIRE(T, kv::ERR_ARG, "t is null");
+        kv::launch_zaxpby(ctx->dev->stream, T->d(), T->d(), nullptr,
+                          make_double2(alpha_re, alpha_im), make_double2(0.0, 0.0), T->lay.total);
+        ctx->dev->sync();
     });
 }
 

@@ -5,6 +5,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "plan.h"
@@ -12,7 +13,8 @@
 
 namespace kv {
 
-enum Prec { PREC_F64 = 0, PREC_F32 = 1 };
+// PREC_BF16X3: FP32 storage, GEMMs as split-BF16 products with FP32 accumulation
+enum Prec { PREC_F64 = 0, PREC_F32 = 1, PREC_BF16X3 = 2 };
 
 // Event pairs recorded around contraction kernels; resolved after the host has
 // synchronised the stream anyway (no extra syncs on the hot path).
@@ -153,15 +155,25 @@ struct DevProduct {
     }
     template <class T2>
     void run(Device& dev, const T2* A, const T2* B, T2* C, double alpha_re = 1.0,
-             double alpha_im = 0.0, EventLog* log = nullptr) {
-        const int cfg = choose<T2>(dev, A, B, C, alpha_re, alpha_im);
+             double alpha_im = 0.0, EventLog* log = nullptr, bool split = false) {
+        const int cfg = split ? 0 : choose<T2>(dev, A, B, C, alpha_re, alpha_im);
         hipEvent_t a = nullptr, b = nullptr;
         if (log) {
             a = log->get();
             b = log->get();
             KV_HIP(hipEventRecord(a, dev.stream));
         }
-        launch<T2>(dev, cfg, A, B, C, alpha_re, alpha_im);
+        if constexpr (std::is_same<T2, float2>::value) {
+            if (split)
+                launch_gemm_bf16x3(dev.stream, p.opA, p.opB, A, B, C, tiles[0].as<GemmTile>(),
+                                   ntiles[0], groups.as<GemmGroup>(), segs.as<GemmSeg>(),
+                                   alpha_re, alpha_im);
+            else
+                launch<T2>(dev, cfg, A, B, C, alpha_re, alpha_im);
+        } else {
+            KV_REQUIRE(!split, ERR_INTERNAL, "split-BF16 GEMM takes FP32 operands");
+            launch<T2>(dev, cfg, A, B, C, alpha_re, alpha_im);
+        }
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
             log->pending.push_back({a, b, p.flop, 0});
@@ -247,6 +259,22 @@ inline std::unique_ptr<Tensor> relayout(const std::shared_ptr<Device>& dev, cons
     return Y;
 }
 
+// y + alpha x over the union of both tensors' blocks (holstein/tensor.h:173-186 appends a
+// block with a new (shape, qn) and axpys into an equal one).
+inline std::unique_ptr<Tensor> axpy(const std::shared_ptr<Device>& dev, const Tensor& X,
+                                    const Tensor& Y, double2 alpha) {
+    KV_REQUIRE(X.lay.rank == Y.lay.rank, ERR_ARG, "axpy: rank mismatch");
+    for (int a = 0; a < X.lay.rank; ++a)
+        KV_REQUIRE(X.lay.dirs[a] == Y.lay.dirs[a], ERR_DIRS, "axpy: leg directions differ");
+    const Layout U = union_layout(X.lay, Y.lay);
+    auto Xu = relayout(dev, X, U);
+    auto out = relayout(dev, Y, U);
+    launch_zaxpby(dev->stream, out->d(), Xu->d(), out->d(), alpha, make_double2(1.0, 0.0),
+                  U.total);
+    dev->sync();
+    return out;
+}
+
 // Effective Hamiltonian on one bond / site, built once per Lanczos call
 // (apply_double / apply_single / apply_zero, holstein/product.h:478-553).
 struct Heff {
@@ -278,7 +306,7 @@ struct Heff {
         t1 = DevBuf(dev->pool.get(), (size_t)S.s1.out.total * es);
         if (nsite >= 1) t2 = DevBuf(dev->pool.get(), (size_t)S.s2.out.total * es);
         if (nsite == 2) t3 = DevBuf(dev->pool.get(), (size_t)S.s3.out.total * es);
-        if (p == PREC_F32) {
+        if (p != PREC_F64) {
             L32 = DevBuf(dev->pool.get(), (size_t)Lt.lay.total * sizeof(float2));
             R32 = DevBuf(dev->pool.get(), (size_t)Rt.lay.total * sizeof(float2));
             x32 = DevBuf(dev->pool.get(), (size_t)S.xl.total * sizeof(float2));
@@ -291,8 +319,9 @@ struct Heff {
     double flop() const { return S.flop; }
 
     template <class T2>
-    void chain(const T2* Ld, const T2* Rd, const T2* x, T2* y, double alpha, EventLog* log) {
-        s1.run<T2>(*dev, Ld, x, t1.as<T2>(), 1.0, 0.0, log);
+    void chain(const T2* Ld, const T2* Rd, const T2* x, T2* y, double alpha, EventLog* log,
+               bool split = false) {
+        s1.run<T2>(*dev, Ld, x, t1.as<T2>(), 1.0, 0.0, log, split);
         const T2* last = t1.as<T2>();
         if (S.nsite >= 1) {
             s2.run<T2>(*dev, last, t2.as<T2>(), log);
@@ -302,7 +331,7 @@ struct Heff {
             s3.run<T2>(*dev, last, t3.as<T2>(), log);
             last = t3.as<T2>();
         }
-        s4.run<T2>(*dev, last, Rd, y, alpha, 0.0, log);
+        s4.run<T2>(*dev, last, Rd, y, alpha, 0.0, log, split);
     }
     // y = alpha * H_eff x, x and y FP64 slabs in layout() order; force64 runs the
     // contractions in FP64 whatever the plan's precision
@@ -313,7 +342,7 @@ struct Heff {
         } else {
             launch_f64_to_f32(dev->stream, x, x32.as<float2>(), S.xl.total);
             chain<float2>(L32.as<float2>(), R32.as<float2>(), x32.as<float2>(), y32.as<float2>(),
-                          1.0, log);
+                          1.0, log, prec == PREC_BF16X3);
             launch_f32_to_f64(dev->stream, y32.as<float2>(), y, alpha, S.xl.total);
         }
     }

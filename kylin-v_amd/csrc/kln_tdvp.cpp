@@ -6,7 +6,7 @@
 //   Krylov tolerance 1e-8 and 15 iterations fixed (src/kln-tdvp.cpp:27-28).
 // After every sweep the MPS is written to State/T-<sweep>.mps in the reference's binary
 // format (holstein/tdvp.h:325-338) and "Max bond dimension = D on bond i-j" is printed
-// (holstein/state.h:187-200).  Extra flags: -p fp64|fp32 (H_eff contraction precision),
+// (holstein/state.h:187-200).  Extra flags: -p fp64|fp32|bf16x3 (H_eff contraction precision),
 // -g device.  The global-Krylov (nK) and SA-1TDVP (nSA) stages are not part of this engine
 // and are rejected.
 #include <getopt.h>
@@ -64,12 +64,16 @@ int main(int argc, char** argv) {
             case 'v': dt = std::atof(optarg); break;
             case 'D': maxdim = std::atoi(optarg); break;
             case 't': tol = std::atof(optarg); break;
-            case 'p': prec = std::strcmp(optarg, "fp32") == 0 ? KV_FP32 : KV_FP64; break;
+            case 'p':
+                prec = std::strcmp(optarg, "fp32") == 0     ? KV_FP32
+                       : std::strcmp(optarg, "bf16x3") == 0 ? KV_BF16X3
+                                                            : KV_FP64;
+                break;
             case 'g': device = std::atoi(optarg); break;
             case 'h':
                 std::printf("Program to calculate tDMRG (TDVP) on an MI355X\n"
                             "Usage: kln-tdvp -w [MPS file] -o [MPO file] -s [n_K,n_2,n_sa,n_1] "
-                            "-v [dt] -D [max bond] -t [SVD cutoff] [-p fp64|fp32] [-g device]\n");
+                            "-v [dt] -D [max bond] -t [SVD cutoff] [-p fp64|fp32|bf16x3] [-g device]\n");
                 return 0;
             default: std::printf("Unknown Option!\n"); break;
         }

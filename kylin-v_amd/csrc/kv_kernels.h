@@ -15,6 +15,10 @@ void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2*
                  const GemmTile* tiles, int ntiles, const GemmGroup* groups,
                  const GemmSeg* segs, double alpha_re, double alpha_im, int cfg = 0);
 bool gemm_config_supports(int cfg, int opA, int opB);
+// Split-BF16 variant on the 64x64 tile list (config 0): FP32 complex operands and output.
+void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const float2* B,
+                        float2* C, const GemmTile* tiles, int ntiles, const GemmGroup* groups,
+                        const GemmSeg* segs, double alpha_re, double alpha_im);
 
 template <class T2>
 void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,
@@ -48,6 +52,8 @@ void launch_scale_invsqrt_dev(hipStream_t st, double2* dst, const double2* src,
                               const double* nsq, int64_t n);
 // dst = src * s
 void launch_scale(hipStream_t st, double2* dst, const double2* src, double s, int64_t n);
+void launch_zaxpby(hipStream_t st, double2* out, const double2* x, const double2* y, double2 a,
+                   double2 b, int64_t n);
 // out = sum_i c[i] * V[i*n + e],  c on the device
 void launch_lincomb(hipStream_t st, double2* out, const double2* V, const double2* c, int nv,
                     int64_t n);

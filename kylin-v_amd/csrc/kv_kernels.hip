@@ -229,6 +229,213 @@ template void launch_gemm<float2>(hipStream_t, int, int, const float2*, const fl
                                   const GemmSeg*, double, double, int);
 
 // ------------------------------------------------------------------------------------
+// grouped complex GEMM, split-BF16 (FP32 operands and accumulation)
+// ------------------------------------------------------------------------------------
+// Every FP32 operand x is split into x_hi = bf16(x) and x_lo = bf16(x - x_hi) while it is
+// staged into LDS; a real product is x_hi*y_hi + x_hi*y_lo + x_lo*y_hi on
+// v_mfma_f32_16x16x32_bf16 (the lo*lo term, ~2^-16 relative, is dropped).  A complex
+// product is four real products, i.e. 12 MFMAs per 16x16x32 complex step: re.re and im.im
+// accumulate separately and are subtracted in the epilogue.  Same tiles, groups and K
+// segments as zgemm_grouped_kernel with the 64x64 tile.
+// LDS holds each operand as four planes (re_hi, re_lo, im_hi, im_lo), K contiguous per row
+// of A and per column of B, so a lane's eight K values are one 16-byte LDS read:
+// v_mfma_f32_16x16x32_bf16 lane l takes A[l%16][8(l/16)..8(l/16)+7] and
+// B[8(l/16)..8(l/16)+7][l%16]; accumulator item v holds D[4(l/16)+v][l%16].
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+__device__ inline void split2(float x0, float x1, bf16x2& hi, bf16x2& lo) {
+    const __bf16 h0 = (__bf16)x0, h1 = (__bf16)x1;
+    hi = bf16x2{h0, h1};
+    lo = bf16x2{(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1)};
+}
+
+template <int OPA, int OPB, int BK>
+__global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
+    const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
+    const GemmTile* __restrict__ tiles, const GemmGroup* __restrict__ groups,
+    const GemmSeg* __restrict__ segs, float alpha_re, float alpha_im) {
+    constexpr int BM = 64, BN = 64, LDK = BK + 8;  // row pitch stays a multiple of 16 bytes
+    __shared__ __attribute__((aligned(16))) __bf16 As[4][BM][LDK];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[4][BN][LDK];
+
+    const GemmTile tile = tiles[blockIdx.x];
+    const GemmGroup g = groups[tile.group];
+    const int m0 = tile.tm * BM, n0 = tile.tn * BN;
+    const int tid = threadIdx.x;
+    const int lane = tid & 63, wave = tid >> 6;
+    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int lr = lane & 15, lk = lane >> 4;
+
+    f32x4 crr[2][2], cii[2][2], cim[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+            crr[i][j] = f32x4{0, 0, 0, 0};
+            cii[i][j] = f32x4{0, 0, 0, 0};
+            cim[i][j] = f32x4{0, 0, 0, 0};
+        }
+
+    for (int s = 0; s < g.seg_count; ++s) {
+        const GemmSeg sg = segs[g.seg_begin + s];
+        const float2* __restrict__ a = A + sg.a_off;
+        const float2* __restrict__ b = B + sg.b_off;
+        const int K = sg.k;
+        for (int k0 = 0; k0 < K; k0 += BK) {
+            // stage A: pairs of consecutive K per thread
+#pragma unroll
+            for (int r = 0; r < (BM * BK / 2) / 256; ++r) {
+                const int e = tid + r * 256;
+                int mm, kk;
+                if (OPA == 0) {
+                    mm = e / (BK / 2);
+                    kk = 2 * (e % (BK / 2));
+                } else {
+                    mm = e % BM;
+                    kk = 2 * (e / BM);
+                }
+                const int gm = m0 + mm, gk = k0 + kk;
+                float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+                if (gm < g.m) {
+                    if (OPA == 0) {
+                        const float2* row = a + (int64_t)gm * sg.lda;
+                        if (gk < K) v0 = row[gk];
+                        if (gk + 1 < K) v1 = row[gk + 1];
+                    } else {
+                        if (gk < K) v0 = a[(int64_t)gk * sg.lda + gm];
+                        if (gk + 1 < K) v1 = a[(int64_t)(gk + 1) * sg.lda + gm];
+                    }
+                }
+                if (OPA == 2) {
+                    v0.y = -v0.y;
+                    v1.y = -v1.y;
+                }
+                bf16x2 h, l;
+                split2(v0.x, v1.x, h, l);
+                *(bf16x2*)&As[0][mm][kk] = h;
+                *(bf16x2*)&As[1][mm][kk] = l;
+                split2(v0.y, v1.y, h, l);
+                *(bf16x2*)&As[2][mm][kk] = h;
+                *(bf16x2*)&As[3][mm][kk] = l;
+            }
+            // stage B
+#pragma unroll
+            for (int r = 0; r < (BN * BK / 2) / 256; ++r) {
+                const int e = tid + r * 256;
+                int nn, kk;
+                if (OPB == 0) {
+                    nn = e % BN;
+                    kk = 2 * (e / BN);
+                } else {
+                    nn = e / (BK / 2);
+                    kk = 2 * (e % (BK / 2));
+                }
+                const int gn = n0 + nn, gk = k0 + kk;
+                float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+                if (gn < g.n) {
+                    if (OPB == 0) {
+                        if (gk < K) v0 = b[(int64_t)gk * sg.ldb + gn];
+                        if (gk + 1 < K) v1 = b[(int64_t)(gk + 1) * sg.ldb + gn];
+                    } else {
+                        const float2* col = b + (int64_t)gn * sg.ldb;
+                        if (gk < K) v0 = col[gk];
+                        if (gk + 1 < K) v1 = col[gk + 1];
+                    }
+                }
+                if (OPB == 2) {
+                    v0.y = -v0.y;
+                    v1.y = -v1.y;
+                }
+                bf16x2 h, l;
+                split2(v0.x, v1.x, h, l);
+                *(bf16x2*)&Bs[0][nn][kk] = h;
+                *(bf16x2*)&Bs[1][nn][kk] = l;
+                split2(v0.y, v1.y, h, l);
+                *(bf16x2*)&Bs[2][nn][kk] = h;
+                *(bf16x2*)&Bs[3][nn][kk] = l;
+            }
+            __syncthreads();
+#pragma unroll
+            for (int kq = 0; kq < BK; kq += 32) {
+                bf16x8 af[2][4], bf[2][4];
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) {
+                        af[i][p] = *(const bf16x8*)&As[p][wm + i * 16 + lr][kq + 8 * lk];
+                        bf[i][p] = *(const bf16x8*)&Bs[p][wn + i * 16 + lr][kq + 8 * lk];
+                    }
+#pragma unroll
+                for (int i = 0; i < 2; ++i)
+#pragma unroll
+                    for (int j = 0; j < 2; ++j) {
+                        f32x4 rr = crr[i][j], ii = cii[i][j], im = cim[i][j];
+                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][0], rr, 0, 0, 0);
+                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][1], rr, 0, 0, 0);
+                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][0], rr, 0, 0, 0);
+                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][2], ii, 0, 0, 0);
+                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][3], ii, 0, 0, 0);
+                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bf[j][2], ii, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][2], im, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][3], im, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][2], im, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][0], im, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][1], im, 0, 0, 0);
+                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bf[j][0], im, 0, 0, 0);
+                        crr[i][j] = rr;
+                        cii[i][j] = ii;
+                        cim[i][j] = im;
+                    }
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int row = m0 + wm + i * 16 + 4 * lk + v;
+                const int col = n0 + wn + j * 16 + lr;
+                if (row < g.m && col < g.n) {
+                    const float re = crr[i][j][v] - cii[i][j][v], im = cim[i][j][v];
+                    float2 o;
+                    o.x = alpha_re * re - alpha_im * im;
+                    o.y = alpha_re * im + alpha_im * re;
+                    C[g.c_off + (int64_t)row * g.ldc + col] = o;
+                }
+            }
+}
+
+void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const float2* B,
+                        float2* C, const GemmTile* tiles, int ntiles, const GemmGroup* groups,
+                        const GemmSeg* segs, double alpha_re, double alpha_im) {
+    if (ntiles <= 0) return;
+    const dim3 grid(ntiles), block(256);
+    const float ar = (float)alpha_re, ai = (float)alpha_im;
+#define KV_BF_CASE(OA, OB)                                                                     \
+    if (opA == OA && opB == OB) {                                                              \
+        hipLaunchKernelGGL((zgemm_bf16x3_kernel<OA, OB, 32>), grid, block, 0, st, A, B, C,     \
+                           tiles, groups, segs, ar, ai);                                       \
+        KV_HIP(hipGetLastError());                                                             \
+        return;                                                                                \
+    }
+    KV_BF_CASE(0, 0)
+    KV_BF_CASE(0, 1)
+    KV_BF_CASE(0, 2)
+    KV_BF_CASE(1, 0)
+    KV_BF_CASE(1, 1)
+    KV_BF_CASE(1, 2)
+    KV_BF_CASE(2, 0)
+    KV_BF_CASE(2, 1)
+    KV_BF_CASE(2, 2)
+#undef KV_BF_CASE
+    throw Error(ERR_ARG, "launch_gemm_bf16x3: bad op codes");
+}
+
+// ------------------------------------------------------------------------------------
 // MPO fold
 // ------------------------------------------------------------------------------------
 template <class T2>
@@ -277,16 +484,21 @@ template <class T2>
 __global__ __launch_bounds__(256) void perm_kernel(const T2* __restrict__ in,
                                                    T2* __restrict__ out,
                                                    const PermTask* __restrict__ tasks) {
-    const PermTask tk = tasks[blockIdx.x];
-    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < tk.size;
+    // fields are read through the pointer: a by-value copy indexed by r would live in
+    // scratch memory
+    const PermTask* tk = tasks + blockIdx.x;
+    const int64_t size = tk->size, in_off = tk->in_off, out_off = tk->out_off;
+    const int rank = tk->rank;
+    for (int64_t e = (int64_t)blockIdx.y * blockDim.x + threadIdx.x; e < size;
          e += (int64_t)gridDim.y * blockDim.x) {
         int64_t rem = e, src = 0;
-        for (int r = tk.rank - 1; r >= 0; --r) {
-            const int64_t idx = rem % tk.oshape[r];
-            rem /= tk.oshape[r];
-            src += idx * tk.istride[r];
+        for (int r = rank - 1; r >= 0; --r) {
+            const int64_t sh = tk->oshape[r];
+            const int64_t idx = rem % sh;
+            rem /= sh;
+            src += idx * tk->istride[r];
         }
-        out[tk.out_off + e] = in[tk.in_off + src];
+        out[out_off + e] = in[in_off + src];
     }
 }
 
@@ -550,6 +762,34 @@ __global__ __launch_bounds__(256) void scale_kernel(double2* __restrict__ dst,
 void launch_scale(hipStream_t st, double2* dst, const double2* src, double s, int64_t n) {
     if (n <= 0) return;
     hipLaunchKernelGGL(scale_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, dst, src, s, n);
+    KV_HIP(hipGetLastError());
+}
+
+// out = a*x + b*y elementwise (complex a, b); x or y may be null (term dropped).
+__global__ __launch_bounds__(256) void zaxpby_kernel(double2* __restrict__ out,
+                                                     const double2* __restrict__ x,
+                                                     const double2* __restrict__ y, double2 a,
+                                                     double2 b, int64_t n) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        double re = 0, im = 0;
+        if (x) {
+            const double2 v = x[e];
+            re += a.x * v.x - a.y * v.y;
+            im += a.x * v.y + a.y * v.x;
+        }
+        if (y) {
+            const double2 v = y[e];
+            re += b.x * v.x - b.y * v.y;
+            im += b.x * v.y + b.y * v.x;
+        }
+        out[e] = make_double2(re, im);
+    }
+}
+
+void launch_zaxpby(hipStream_t st, double2* out, const double2* x, const double2* y, double2 a,
+                   double2 b, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zaxpby_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, out, x, y, a, b, n);
     KV_HIP(hipGetLastError());
 }
 

@@ -30,15 +30,23 @@ namespace kv {
 // 1 MiB granularity above.
 class Pool {
   public:
-    explicit Pool(int device) : device_(device) {}
+    explicit Pool(int device) : device_(device) {
+        if (const char* c = std::getenv("KV_POOL_CACHE_GB")) cap_ = (size_t)(std::atof(c) * (1ull << 30));
+    }
     ~Pool() { release_all(); }
     Pool(const Pool&) = delete;
     Pool& operator=(const Pool&) = delete;
 
+    // Size classes: 256-byte steps below 64 KiB, then four classes per power of two
+    // (waste at most 25%).  Bond dimensions change from bond to bond, so exact-size classes
+    // would keep every shape ever seen cached; geometric classes let neighbouring shapes
+    // share buffers.
     static size_t size_class(size_t bytes) {
         if (bytes == 0) bytes = 1;
-        const size_t g = bytes < (1u << 20) ? 256 : (1u << 20);
-        return (bytes + g - 1) / g * g;
+        if (bytes < (size_t(1) << 16)) return (bytes + 255) / 256 * 256;
+        int k = 63 - __builtin_clzll((unsigned long long)bytes);
+        const size_t step = size_t(1) << (k - 2);
+        return (bytes + step - 1) / step * step;
     }
     void* get(size_t bytes) {
         const size_t sz = size_class(bytes);
@@ -65,6 +73,21 @@ class Pool {
         const size_t sz = size_class(bytes);
         free_[sz].push_back(p);
         cached_ += sz;
+        // bounded cache: runs whose tensor sizes keep changing (chains of different shape
+        // on one context) would otherwise pin device memory in ever new size classes
+        if (cached_ > cap_) trim(cap_ / 2);
+    }
+    // Free cached buffers, largest first, until at most keep bytes stay cached.
+    // hipFree synchronises the device, so freed buffers are no longer in use.
+    void trim(size_t keep) {
+        for (auto it = free_.rbegin(); it != free_.rend() && cached_ > keep; ++it) {
+            while (!it->second.empty() && cached_ > keep) {
+                (void)hipFree(it->second.back());
+                it->second.pop_back();
+                cached_ -= it->first;
+                live_ -= it->first;
+            }
+        }
     }
     void release_cached() {
         for (auto& kvp : free_) {
@@ -83,6 +106,7 @@ class Pool {
     int device_;
     std::map<size_t, std::vector<void*>> free_;
     size_t live_ = 0, cached_ = 0;
+    size_t cap_ = (size_t)24 << 30;  // KV_POOL_CACHE_GB overrides
 };
 
 // Owning device buffer returned to the pool on destruction.

@@ -22,25 +22,39 @@ def main():
     ap.add_argument("--workers", type=int, default=8)
     ap.add_argument("--precision", default="fp64")
     ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--first", type=int, default=0, help="run chains [first, first+count) of the set")
+    ap.add_argument("--count", type=int, default=None)
     args = ap.parse_args()
     import __graft_entry__ as GE
     GE.load_package()
     from kylin_v_amd import batch
 
     specs = batch.c3_chains(args.chains, args.seed)
+    count = len(specs) - args.first if args.count is None else args.count
+    specs = specs[args.first:args.first + count]
     t0 = time.perf_counter()
 
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+
+    def free_gib():
+        free, total = ctypes.c_size_t(), ctypes.c_size_t()
+        if hip.hipMemGetInfo(ctypes.byref(free), ctypes.byref(total)) != 0:
+            return float("nan")
+        return free.value / 2**30
+
     def progress(i, r):
-        print(f"c3: chain {i} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
-              f"{r['seconds']:.1f} s, max bond {max(r['bonds'])}", file=sys.stderr, flush=True)
+        print(f"c3: chain {args.first + i} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
+              f"{r['seconds']:.1f} s, max bond {max(r['bonds'])}, device free {free_gib():.1f} GiB",
+              file=sys.stderr, flush=True)
 
     res, wall = batch.run_batch(specs, args.steps, dt=0.05, workers=args.workers,
                                 precision=args.precision, on_progress=progress)
     out = {"workload": "64 Holstein chains, 2TDVP 50 steps dt=0.05 (seeded set)",
-           "chains": args.chains, "steps": args.steps,
+           "chains": args.chains, "first": args.first, "count": len(specs), "steps": args.steps,
            "workers": min(args.workers, batch.MAX_WORKERS),
            "precision": args.precision, "wall_s": wall,
-           "chain_steps_per_s": args.chains * args.steps / wall,
+           "chain_steps_per_s": len(specs) * args.steps / wall,
            "sum_chain_seconds": sum(r["seconds"] for r in res),
            "max_bond": max(max(r["bonds"]) for r in res)}
     print(json.dumps(out), flush=True)

@@ -107,12 +107,18 @@ def test_grouped_gemm_product_large_ragged_blocks(kv, spec):
     assert max_diff(got, ref) <= 1e-12 * scale(ref)
 
 
+# Bars relative to the contraction of absolute values: FP64 rounding; FP32 rounding over the
+# four chained stages; split-BF16 drops the lo*lo term and rounds lo to 8 bits, ~3*2^-18 of
+# |a||b| per product, over the same stages.
+HEFF_TOL = {"fp64": 1e-12, "fp32": 2e-5, "bf16x3": 2e-4}
+
+
 @pytest.mark.parametrize("layout", ["merged", "pp", "split"])
-@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
 def test_heff_matches_oracle(kv, layout, precision):
     print(f"case heff {layout} {precision}", flush=True)
     lat, mpo, eng = evolved(layout)
-    tol = 1e-12 if precision == "fp64" else 2e-5
+    tol = HEFF_TOL[precision]
     envl, envr = eng.envL[-1], eng.envR[-1]
     gL, gR = up(kv, envl), up(kv, envr)
     gW = [up(kv, w) for w in mpo]
@@ -129,6 +135,30 @@ def test_heff_matches_oracle(kv, layout, precision):
     y0 = T.apply_zero(envl1, nex, envr)
     got0 = down(kv.apply_zero(up(kv, envl1), up(kv, nex), gR, precision))
     assert max_diff(got0, y0) <= tol * rounding_scale(T.apply_zero, envl1, nex, envr)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
+def test_heff_large_ragged_sectors(kv, precision):
+    """One- and two-site H_eff on random block tensors whose sector dims straddle the 64x64
+    output tile and the 32-deep split-BF16 K step (edge tiles, K tails, several K segments)."""
+    print(f"case heff ragged {precision}", flush=True)
+    rng = np.random.default_rng(5)
+    a_l = {0: 70, 1: 33}
+    s_l = {0: 5, 1: 3}
+    w_l = {-1: 1, 0: 2, 1: 1}
+    b_l = {0: 45, 1: 97}
+    L = rand_block_tensor(rng, (1, -1, -1), [a_l, w_l, a_l])
+    W = rand_block_tensor(rng, (1, 1, -1, -1), [s_l, w_l, s_l, w_l])
+    R = rand_block_tensor(rng, (-1, 1, 1), [b_l, w_l, b_l])
+    x = rand_block_tensor(rng, (1, 1, -1), [a_l, s_l, b_l])
+    tol = HEFF_TOL[precision]
+    y = T.apply_single(L, x, W, R)
+    got = down(kv.apply_single(up(kv, L), up(kv, x), up(kv, W), up(kv, R), precision))
+    assert max_diff(got, y) <= tol * rounding_scale(T.apply_single, L, x, W, R)
+    x2 = rand_block_tensor(rng, (1, 1, 1, -1), [a_l, s_l, s_l, b_l])
+    y2 = T.apply_double(L, x2, W, W, R)
+    got2 = down(kv.apply_double(up(kv, L), up(kv, x2), up(kv, W), up(kv, W), up(kv, R), precision))
+    assert max_diff(got2, y2) <= tol * rounding_scale(T.apply_double, L, x2, W, W, R)
 
 
 @pytest.mark.parametrize("layout", ["merged", "pp"])
@@ -285,20 +315,21 @@ def test_two_site_tdvp_matches_exact_diagonalisation(kv):
         assert abs(kv.overlap_states(ch.psi0, eng.mps) - ex) < 1e-10
 
 
-def test_mixed_precision_tdvp_tracks_fp64(kv):
-    print("case tdvp fp32", flush=True)
+@pytest.mark.parametrize("precision,bar", [("fp32", 1e-5), ("bf16x3", 1e-4)])
+def test_mixed_precision_tdvp_tracks_fp64(kv, precision, bar):
+    print(f"case tdvp {precision}", flush=True)
     g = np.array([1.0, 0.8, 1.2, 0.9])
     from kylin_v_amd import model as KM
     ch = KM.HolsteinChain(4, [1.0], g, [4], 1.0, "merged")
     e64 = kv.TDVP(ch.psi0, ch.mpo, cutoff=1e-10, maxdim=24, dt=0.05, precision="fp64")
-    e32 = kv.TDVP(ch.psi0, ch.mpo, cutoff=1e-10, maxdim=24, dt=0.05, precision="fp32")
+    e32 = kv.TDVP(ch.psi0, ch.mpo, cutoff=1e-10, maxdim=24, dt=0.05, precision=precision)
     dmax = 0.0
     for _ in range(8):
         e64.two_site()
         e32.two_site()
         dmax = max(dmax, abs(kv.overlap_states(ch.psi0, e64.mps) - kv.overlap_states(ch.psi0, e32.mps)))
-    print(f"fp32 vs fp64 max|dC| = {dmax:.3e}", flush=True)
-    assert dmax < 1e-5
+    print(f"{precision} vs fp64 max|dC| = {dmax:.3e}", flush=True)
+    assert dmax < bar
 
 
 def test_mps_mpo_files_round_trip(kv, tmp_path):
