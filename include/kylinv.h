@@ -96,6 +96,16 @@ int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im);
 int kv_product(kv_ctx* ctx, const kv_tensor* m1, const kv_tensor* m2, int nc,
                char op1, char op2, int precision, kv_tensor** out);
 
+/* Profiling form of kv_heff_apply: builds H_eff once, applies it once untimed (tile
+ * autotuning), then `reps` times timed by HIP events on the context stream.
+ * out[8] = {seconds per apply, algorithmic flop per apply, GEMM kernel seconds per apply,
+ * GEMM flop per apply, GEMM launches per apply, MPO-fold seconds per apply, MPO-fold flop
+ * per apply, MPO-fold launches per apply}.  No reference counterpart (the reference's
+ * Timer brackets whole modules, holstein/tdvp.h:384). */
+int kv_heff_profile(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                    const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x, int precision,
+                    int reps, double* out);
+
 /* Tensor algebra used by the TDVP adjoint.
  * kv_tensor_transpose: transpose(t, axes) (holstein/tensor.h:243-260), out leg a = t leg axes[a].
  * kv_tensor_axpy: out = y + alpha x over the union of both block sets (operator+=,

@@ -75,6 +75,7 @@ SIGNATURES = {
     "kv_tensor_norm": (_I, [_P, _DP]),
     "kv_tensor_overlap": (_I, [_P, _P, _DP]),
     "kv_product": (_I, [_P, _P, _P, _I, ctypes.c_char, ctypes.c_char, _I, _PP]),
+    "kv_heff_profile": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _I, _DP]),
     "kv_tensor_transpose": (_I, [_P, _P, _IP, _PP]),
     "kv_tensor_axpy": (_I, [_P, _P, _P, _D, _D, _PP]),
     "kv_tensor_scale": (_I, [_P, _P, _D, _D]),
@@ -303,6 +304,19 @@ def _heff(nsite, L, W1, W2, R, x, precision):
                                     W2.h if W2 is not None else None, R.h, x.h,
                                     _PREC[precision], ctypes.byref(h)))
     return Tensor(ctx, h)
+
+
+def heff_profile(nsite, envL, x, mpo1, mpo2, envR, precision="fp64", reps=5):
+    """Times `reps` applications of one H_eff (kv_heff_profile); returns a dict of per-apply
+    seconds, flop and launch counts."""
+    ctx = x.ctx
+    out = (ctypes.c_double * 8)()
+    ctx.check(ctx.lib.kv_heff_profile(ctx.h, nsite, envL.h, mpo1.h if mpo1 is not None else None,
+                                      mpo2.h if mpo2 is not None else None, envR.h, x.h,
+                                      _PREC[precision], reps, out))
+    keys = ("apply_s", "flop", "gemm_s", "gemm_flop", "gemm_launches", "wfold_s", "wfold_flop",
+            "wfold_launches")
+    return dict(zip(keys, list(out)))
 
 
 def apply_double(envL, x, mpo1, mpo2, envR, precision="fp64"):

@@ -431,3 +431,33 @@ class OneSiteAdjoint:
     def value_and_grad(self, theta, on_step=None):
         c = self.forward(theta, on_step)
         return abs(c) ** 2, self.backward(on_step)
+
+
+def _autograd_function():
+    import torch
+
+    class TdvpAutocorr(torch.autograd.Function):
+        """|C(T)|^2 of a one-site TDVP run as a function of theta = (g_1..g_N, J, omega)
+        (float64 tensor).  forward runs the device forward pass and keeps the per-step
+        checkpoints on the engine; backward runs the hand-written adjoint."""
+
+        @staticmethod
+        def forward(ctx, theta, engine):
+            c = engine.forward(theta.detach().cpu().numpy().astype(np.float64))
+            ctx.engine = engine
+            return torch.tensor(abs(c) ** 2, dtype=torch.float64)
+
+        @staticmethod
+        def backward(ctx, gout):
+            grad = torch.as_tensor(ctx.engine.backward(), dtype=torch.float64)
+            return gout * grad, None
+
+    return TdvpAutocorr
+
+
+def __getattr__(name):
+    if name == "TdvpAutocorr":
+        fn = _autograd_function()
+        globals()["TdvpAutocorr"] = fn
+        return fn
+    raise AttributeError(name)

@@ -272,6 +272,40 @@ int kv_heff_apply(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W
     });
 }
 
+int kv_heff_profile(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                    const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x, int precision,
+                    int reps, double* out) {
+    if (!ctx || !out || reps < 1) return KV_ERR_ARG;
+    return guard(ctx, [&] {
+        kv::Tensor* w1 = nsite >= 1 ? need(W1, "W1") : nullptr;
+        kv::Tensor* w2 = nsite == 2 ? need(W2, "W2") : nullptr;
+        kv::Heff H;
+        H.build(ctx->dev, nsite, *need(L, "L"), w1, w2, *need(R, "R"), need(x, "x")->lay,
+                prec_of(precision));
+        auto xin = kv::relayout(ctx->dev, *x->t, H.layout());
+        auto y = kv::Tensor::create(ctx->dev, H.layout(), false);
+        H.apply(xin->d(), y->d(), 1.0);  // first call: GEMM tile autotuning, not timed
+        ctx->dev->sync();
+        kv::EventLog log;
+        kv::KernelStats st;
+        kv::EventTimer ev;
+        ev.start(ctx->dev->stream);
+        for (int r = 0; r < reps; ++r) H.apply(xin->d(), y->d(), 1.0, &log);
+        ev.stop(ctx->dev->stream);
+        const double wall_ms = ev.ms();
+        ctx->dev->sync();
+        log.resolve(st);
+        out[0] = 1e-3 * wall_ms / reps;
+        out[1] = H.flop();
+        out[2] = 1e-3 * st.gemm_ms / reps;
+        out[3] = st.gemm_flop / reps;
+        out[4] = (double)st.gemm_launches / reps;
+        out[5] = 1e-3 * st.wfold_ms / reps;
+        out[6] = st.wfold_flop / reps;
+        out[7] = (double)st.wfold_launches / reps;
+    });
+}
+
 int kv_env_update(kv_ctx* ctx, const kv_tensor* env, const kv_tensor* bra, const kv_tensor* W,
                   const kv_tensor* ket, int left_to_right, int precision, kv_tensor** out) {
     if (!ctx || !out) return KV_ERR_ARG;

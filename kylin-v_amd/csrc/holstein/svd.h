@@ -11,6 +11,15 @@
 //    MI355X against ~2 s for the Jacobi SVD of the 2048 x 4096 sector.  Eigenvalues of G
 //    carry absolute error ~ n eps lambda_max, so a sector whose smallest kept singular
 //    value falls below guard * s_max is recomputed on the exact route.
+//  * isometric-side Gram (sectors whose isometric side is at most twice the other):
+//    G = M M^H when the left factor is the isometry (S goes right), M^H M when the right
+//    one is; the eigenvectors of G are the isometry itself, orthonormal to machine
+//    precision, and the other factor is one GEMM (U^H M or M V) that already carries S,
+//    so no division by small singular values.  Eigenvalues of G are only accurate to
+//    ~eps lambda_max, so when the truncation reaches below tail * s_max the tail
+//    eigenvectors (an accurate subspace as a whole) are refined by a second Gram pass on
+//    M restricted to that subspace, whose norm is below tail * s_max: the tail's singular
+//    values then carry absolute error ~eps (tail s_max)^2 / s.
 // Layout of the factors: U_M (rows x k) and V_M^H (k x cols) row-major.  rocSOLVER is
 // column-major, so the exact route hands it M^T (cols x rows): with M^T = U' S V'^H,
 // M = conj(V') S U'^T, so rocSOLVER's V^H output read row-major (ld k) is U_M and its U
@@ -66,12 +75,16 @@ struct SvdResult {
 struct SvdOptions {
     int gram_min = 512;     // smaller side above which the Gram route is used
     double guard = 1e-5;    // Gram route only if s_kept_min >= guard * s_max
+    int iso_min = 64;       // smaller side from which the isometric-side Gram route is used
+    double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
     static SvdOptions from_env() {
         SvdOptions o;
         if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
         if (const char* m = std::getenv("KV_GRAM_MIN")) o.gram_min = std::atoi(m);
+        if (const char* m = std::getenv("KV_ISO_MIN")) o.iso_min = std::atoi(m);
+        if (const char* t = std::getenv("KV_ISO_TAIL")) o.tail = std::atof(t);
         return o;
     }
 };
@@ -102,9 +115,12 @@ inline void gemm_dense(Device& D, char opA, char opB, const double2* A, int lda,
 
 struct SectorSvd {
     int64_t nr = 0, nc = 0;
-    int k = 0;            // number of singular values available
+    int k = 0;            // number of singular values available (Gram: side of G)
     bool gram = false;    // values from the Gram eigenproblem, vectors not yet formed
+    bool iso = false;     // isometric-side Gram: the eigenvectors are the isometry
+    bool refined = false; // isometric route: tail already refined
     bool colside = true;  // Gram of M^H M (else M M^H)
+    bool s_absorbed = false;  // the non-isometric factor already carries S
     DevBuf M, G, S, U, Vh;  // U: nr x ldU, Vh: k x nc
     int ldU = 0;
     std::vector<double> s;  // descending
@@ -149,9 +165,11 @@ inline void exact_sector(Device& D, Solver& sol, SectorSvd& X, bool keep_zero) {
     KV_REQUIRE(hinfo == 0, ERR_SOLVER, "Problems in SVD (did not converge)");
 }
 
-// Gram matrix of a sector (G = M^H M when cols <= rows, else M M^H).
-inline void gram_prepare(Device& D, SectorSvd& X) {
-    X.colside = X.nc <= X.nr;
+// Gram matrix of a sector (G = M^H M when cols <= rows, else M M^H; the isometric route
+// takes the side given by colside).
+inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside = true) {
+    X.iso = iso;
+    X.colside = iso ? colside : X.nc <= X.nr;
     const int n = (int)(X.colside ? X.nc : X.nr);
     X.k = n;
     X.gram = true;
@@ -267,6 +285,90 @@ inline void gram_vectors(Device& D, SectorSvd& X, int nk) {
     D.sync();
 }
 
+// Isometric route, second pass over the tail: the eigenvectors of G with values below
+// tail * s_max (buffer rows [0, nt) in ascending order) span an accurate subspace; M
+// restricted to it (Mt = Bt M or M Bt^H) has norm ~ tail * s_max, and the eigenvectors Y
+// of its own Gram matrix rotate the tail rows: Bt <- Y^H Bt.
+inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
+    X.refined = true;
+    const int n = X.k;
+    const int kmax = (int)std::min(X.nr, X.nc);
+    const double smax = X.s.empty() ? 0.0 : X.s[0];
+    int na = 0;
+    while (na < kmax && X.s[na] >= tail * smax) ++na;
+    const int nt = n - na;
+    if (nt <= 0 || na >= kmax) return;
+    double2* Bt = X.G.as<double2>();  // rows [0, nt) of the eigenvector buffer (ld n)
+    const double2* m = X.M.as<double2>();
+    SectorSvd T;
+    T.k = nt;
+    T.gram = true;
+    T.G = DevBuf(D.pool.get(), (size_t)nt * nt * sizeof(double2));
+    if (!X.colside) {  // rows side: Mt = Bt M (nt x nc), Gt = Mt Mt^H
+        DevBuf Mt(D.pool.get(), (size_t)nt * X.nc * sizeof(double2));
+        gemm_dense(D, 'N', 'N', Bt, n, m, (int)X.nc, Mt.as<double2>(), (int)X.nc, nt, (int)X.nc,
+                   (int)X.nr);
+        gemm_dense(D, 'N', 'C', Mt.as<double2>(), (int)X.nc, Mt.as<double2>(), (int)X.nc,
+                   T.G.as<double2>(), nt, nt, nt, (int)X.nc);
+    } else {  // columns side: Mt = M Bt^H (nr x nt), Gt = Mt^H Mt
+        DevBuf Mt(D.pool.get(), (size_t)X.nr * nt * sizeof(double2));
+        gemm_dense(D, 'N', 'C', m, (int)X.nc, Bt, n, Mt.as<double2>(), nt, (int)X.nr, nt,
+                   (int)X.nc);
+        gemm_dense(D, 'C', 'N', Mt.as<double2>(), nt, Mt.as<double2>(), nt, T.G.as<double2>(),
+                   nt, nt, nt, (int)X.nr);
+    }
+    gram_solve(D, sol, {&T});
+    // Bt <- Ybuf Bt (rows y_j^H, ascending) and the tail values replaced
+    DevBuf Bn(D.pool.get(), (size_t)nt * n * sizeof(double2));
+    gemm_dense(D, 'N', 'N', T.G.as<double2>(), nt, Bt, n, Bn.as<double2>(), n, nt, n, nt);
+    KV_HIP(hipMemcpyAsync(Bt, Bn.get(), (size_t)nt * n * sizeof(double2), hipMemcpyDeviceToDevice,
+                          D.stream));
+    D.sync();
+    for (int j = 0; j < kmax - na; ++j) X.s[na + j] = T.s[j];  // the rest is null space
+}
+
+// Isometric route: the nk leading vectors are rows of the eigenvector buffer; the other
+// factor is one GEMM and carries S.
+inline void iso_vectors(Device& D, SectorSvd& X, int nk) {
+    const int n = X.k;
+    DevBuf B(D.pool.get(), (size_t)nk * n * sizeof(double2));  // descending rows
+    {
+        std::vector<Copy2D> cps(nk);
+        for (int j = 0; j < nk; ++j) {
+            Copy2D c{};
+            c.src_off = (int64_t)(n - 1 - j) * n;
+            c.dst_off = (int64_t)j * n;
+            c.rows = 1;
+            c.cols = n;
+            c.src_ld = n;
+            c.dst_ld = n;
+            cps[j] = c;
+        }
+        DevBuf dc = upload_vec(D, cps);
+        launch_copy2d<double2>(D.stream, X.G.as<double2>(), B.as<double2>(), dc.as<Copy2D>(), nk, 1,
+                               nullptr);
+    }
+    X.s_absorbed = true;
+    X.S = upload_vec(D, std::vector<double>(X.s.begin(), X.s.begin() + nk));
+    X.ldU = nk;
+    X.U = DevBuf(D.pool.get(), (size_t)(X.nr * nk) * sizeof(double2));
+    X.Vh = DevBuf(D.pool.get(), (size_t)(nk * X.nc) * sizeof(double2));
+    const double2* m = X.M.as<double2>();
+    if (!X.colside) {
+        // U_M = B^H (nr x nk); S V^H = B M (nk x nc)
+        launch_ctranspose(D.stream, B.as<double2>(), X.U.as<double2>(), nk, n, n, nk);
+        gemm_dense(D, 'N', 'N', B.as<double2>(), n, m, (int)X.nc, X.Vh.as<double2>(), (int)X.nc,
+                   nk, (int)X.nc, n);
+    } else {
+        // V^H = B (nk x nc); U S = M B^H (nr x nk)
+        KV_HIP(hipMemcpyAsync(X.Vh.get(), B.get(), (size_t)nk * n * sizeof(double2),
+                              hipMemcpyDeviceToDevice, D.stream));
+        gemm_dense(D, 'N', 'C', m, (int)X.nc, B.as<double2>(), n, X.U.as<double2>(), nk,
+                   (int)X.nr, nk, (int)X.nc);
+    }
+    D.sync();
+}
+
 inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, const Tensor& A,
                               int r1, double cutoff, int maxdim, bool left2right,
                               int left_dir = -1, const SvdOptions& opt = SvdOptions::from_env()) {
@@ -309,9 +411,15 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
                                    (int)cps.size(), maxrows, nullptr);
             D.sync();
         }
+        const int64_t iso_side = left2right ? rs.len : cs.len;
+        const int64_t oth_side = left2right ? cs.len : rs.len;
+        const bool iso = !opt.force_exact && std::min(rs.len, cs.len) >= opt.iso_min &&
+                         iso_side <= 2 * oth_side;
         const bool gram = !opt.force_exact && cutoff >= 0.0 &&
                           std::min(rs.len, cs.len) > opt.gram_min;
-        if (gram)
+        if (iso)
+            gram_prepare(D, X[p], true, !left2right);
+        else if (gram)
             gram_prepare(D, X[p]);
         else
             exact_sector(D, sol, X[p], cutoff < 0.0);
@@ -322,25 +430,42 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         for (int p = 0; p < np; ++p)
             if (X[p].gram) by_n[X[p].k].push_back(&X[p]);
         for (auto& kvp : by_n) gram_solve(D, sol, kvp.second);
+        // the isometric route's values: only min(rows, cols) are singular values
+        for (int p = 0; p < np; ++p)
+            if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
     }
     // phase 2: global truncation; Gram sectors that would keep values too small for the
     // Gram eigenvalues' absolute accuracy move to the exact route and the rule reruns
     std::vector<int> kept;
-    for (int pass = 0; pass < 2; ++pass) {
+    for (int pass = 0; pass < 3; ++pass) {
         std::vector<std::vector<double>> sv(np);
         for (int p = 0; p < np; ++p) sv[p] = X[p].s;
         kept = truncation_counts(sv, cutoff, maxdim);
-        bool redo = false;
+        // smallest kept value overall: an isometric-route tail that may reach it is refined
+        double s_last = -1.0;
         for (int p = 0; p < np; ++p)
-            if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
+            if (kept[p] > 0 && (s_last < 0 || X[p].s[kept[p] - 1] < s_last)) s_last = X[p].s[kept[p] - 1];
+        bool redo = false;
+        for (int p = 0; p < np; ++p) {
+            if (X[p].iso) {
+                if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
+                    opt.tail * X[p].s[0] >= s_last) {
+                    iso_refine(D, sol, X[p], opt.tail);
+                    redo = true;
+                }
+            } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
                 exact_sector(D, sol, X[p], false);
                 redo = true;
             }
+        }
         if (!redo) break;
     }
     for (int p = 0; p < np; ++p) {
         res.total += (int)X[p].s.size();
-        if (X[p].gram) {
+        if (X[p].iso) {
+            res.gram_sectors += 1;
+            if (kept[p] > 0) iso_vectors(D, X[p], kept[p]);
+        } else if (X[p].gram) {
             res.gram_sectors += 1;
             if (kept[p] > 0) gram_vectors(D, X[p], kept[p]);
         } else {
@@ -406,7 +531,7 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             c.cols = nk;
             c.src_ld = X[p].ldU;
             c.dst_ld = nk;
-            c.scale_mode = left2right ? 0 : 2;  // U S when S goes left
+            c.scale_mode = (left2right || X[p].s_absorbed) ? 0 : 2;  // U S when S goes left
             maxr_l = std::max(maxr_l, c.rows);
             lc.push_back(c);
         }
@@ -422,7 +547,7 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             c.cols = (int)csub.size;
             c.src_ld = (int)nc;
             c.dst_ld = (int)csub.size;
-            c.scale_mode = left2right ? 1 : 0;  // S V^H when S goes right
+            c.scale_mode = (left2right && !X[p].s_absorbed) ? 1 : 0;  // S V^H when S goes right
             maxr_r = std::max(maxr_r, c.rows);
             rc.push_back(c);
         }

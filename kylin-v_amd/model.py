@@ -264,3 +264,119 @@ def synthetic_state_blocks(sites, D, seed=0, decay=1.0, Q=1):
         out[i] = blocks
         right = left
     return out
+
+
+def _bond_dims(blocks, axis):
+    out = {}
+    for q, b in blocks.items():
+        out[q[axis]] = b.shape[axis]
+    return out
+
+
+def canonical_blocks(mps, rel_cutoff=1e-13, maxdim=None):
+    """Right-canonical form of an MPS given as {qn: block} dicts, with every bond cut to its
+    Schmidt rank (singular values below rel_cutoff * the bond's largest are dropped) and to
+    at most maxdim values per bond (the largest over all sectors).  A left-to-right SVD
+    sweep cuts the bonds, a right-to-left sweep leaves sites 1..N-1 right-isometric and the
+    norm on site 0, which is then normalised."""
+    mps = [dict(b) for b in mps]
+    ns = len(mps)
+    for i in range(ns - 1):  # left to right: rows (a, s) grouped by right charge
+        rows = {}
+        for (qa, qs, qb), b in mps[i].items():
+            rows.setdefault(qb, []).append((qa, qs))
+        svd = {}
+        for qb, keys in rows.items():
+            keys.sort()
+            mats = [mps[i][k + (qb,)].reshape(-1, mps[i][k + (qb,)].shape[2]) for k in keys]
+            svd[qb] = (keys, mats) + tuple(np.linalg.svd(np.vstack(mats), full_matrices=False))
+        alls = np.sort(np.concatenate([v[3] for v in svd.values()]))[::-1]
+        thr = rel_cutoff * max(alls[0], 1e-300)
+        if maxdim is not None and len(alls) > maxdim:
+            thr = max(thr, alls[maxdim])
+        new_i, carry = {}, {}
+        for qb, (keys, mats, u, s, vh) in svd.items():
+            k = int(np.sum(s > thr))
+            if k == 0:
+                continue
+            pos = 0
+            for key, m in zip(keys, mats):
+                sh = mps[i][key + (qb,)].shape
+                new_i[key + (qb,)] = u[pos:pos + m.shape[0], :k].reshape(sh[0], sh[1], k)
+                pos += m.shape[0]
+            carry[qb] = s[:k, None] * vh[:k]
+        mps[i] = new_i
+        nxt = {}
+        for (qa, qs, qb), b in mps[i + 1].items():
+            if qa in carry:
+                nxt[(qa, qs, qb)] = np.tensordot(carry[qa], b, axes=(1, 0))
+        mps[i + 1] = nxt
+    for i in range(ns - 1, 0, -1):  # right to left: columns (s, b) grouped by left charge
+        cols = {}
+        for (qa, qs, qb), b in mps[i].items():
+            cols.setdefault(qa, []).append((qs, qb))
+        new_i, carry = {}, {}
+        for qa, keys in cols.items():
+            keys.sort()
+            mats = [mps[i][(qa,) + k].reshape(mps[i][(qa,) + k].shape[0], -1) for k in keys]
+            u, s, vh = np.linalg.svd(np.hstack(mats), full_matrices=False)
+            k = int(np.sum(s > rel_cutoff * max(s[0], 1e-300)))
+            pos = 0
+            for key, m in zip(keys, mats):
+                sh = mps[i][(qa,) + key].shape
+                new_i[(qa,) + key] = vh[:k, pos:pos + m.shape[1]].reshape(k, sh[1], sh[2])
+                pos += m.shape[1]
+            carry[qa] = u[:, :k] * s[None, :k]
+        mps[i] = new_i
+        prv = {}
+        for (qa, qs, qb), b in mps[i - 1].items():
+            if qb in carry:
+                prv[(qa, qs, qb)] = np.tensordot(b, carry[qb], axes=(2, 0))
+        mps[i - 1] = prv
+    nrm = np.sqrt(sum(np.vdot(b, b).real for b in mps[0].values()))
+    mps[0] = {q: b / nrm for q, b in mps[0].items()}
+    return mps
+
+
+def expanded_state_blocks(sites, init, D, seed=0, eps=0.05, decay=1.0):
+    """The product state of `init` expanded to bond dimension D with full-rank Schmidt
+    spectra: the direct sum of the product MPS and eps times a seeded random MPS of the same
+    charge with bonds up to D, brought to right-canonical form with at most D values kept
+    per bond.  Every bond then carries
+    the product state's Schmidt value (~1) plus a seeded tail of weight ~eps, which a run at
+    fixed bond dimension (one-site TDVP) needs to be well conditioned."""
+    P = product_state_blocks(sites, init)
+    Q = max(q[2] for q in P[-1])
+    R = synthetic_state_blocks(sites, D, seed=seed, decay=decay, Q=Q)
+    ns = len(sites)
+    out = []
+    for i in range(ns):
+        pl, pr = _bond_dims(P[i], 0), _bond_dims(P[i], 2)
+        rl, rr = _bond_dims(R[i], 0), _bond_dims(R[i], 2)
+        first, last = i == 0, i == ns - 1
+        blocks = {}
+        keys = set(P[i]) | set(R[i])
+        for (qa, qs, qb) in keys:
+            ds = len(sites[i].sec[qs])
+            da = 1 if first else pl.get(qa, 0) + rl.get(qa, 0)
+            db = 1 if last else pr.get(qb, 0) + rr.get(qb, 0)
+            blk = np.zeros((da, ds, db), dtype=np.complex128)
+            p, r = P[i].get((qa, qs, qb)), R[i].get((qa, qs, qb))
+            if first:
+                if p is not None:
+                    blk[:, :, :p.shape[2]] = p
+                if r is not None:
+                    blk[:, :, pr.get(qb, 0):] = eps * r
+            elif last:
+                if p is not None:
+                    blk[:p.shape[0]] = p
+                if r is not None:
+                    blk[pl.get(qa, 0):] = r
+            else:
+                if p is not None:
+                    blk[:p.shape[0], :, :p.shape[2]] = p
+                if r is not None:
+                    blk[pl.get(qa, 0):, :, pr.get(qb, 0):] = r
+            blocks[(qa, qs, qb)] = blk
+        out.append(blocks)
+    return canonical_blocks(out, maxdim=D)

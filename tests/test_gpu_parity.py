@@ -396,3 +396,53 @@ def test_svd_gram_route_matches_oracle(kv):
         gram = oproduct(iso, iso, 2, "C", "N") if l2r else oproduct(iso, iso, 2, "N", "C")
         for q, blk in gram.blocks.items():
             np.testing.assert_allclose(blk, np.eye(blk.shape[0]), atol=1e-9)
+
+
+def _spectrum_tensor(rng, svals_by_q):
+    """Rank-3 tensor (a, s, b), dirs (+1,+1,-1), whose row sector q (rows (a, s) with
+    qa + qs = q, columns b of charge q) is a dense matrix with singular values svals_by_q[q]."""
+    a_l, s_l, b_l = {0: 40, 1: 30}, {0: 3, 1: 2}, {0: 100, 1: 120}
+    t = OT((1, 1, -1))
+    for q, sv in svals_by_q.items():
+        keys = [(qa, qs) for qa in a_l for qs in s_l if qa + qs == q]
+        nrow = sum(a_l[qa] * s_l[qs] for qa, qs in keys)
+        ncol = b_l[q]
+        u, _ = np.linalg.qr(rng.standard_normal((nrow, nrow)) + 1j * rng.standard_normal((nrow, nrow)))
+        v, _ = np.linalg.qr(rng.standard_normal((ncol, ncol)) + 1j * rng.standard_normal((ncol, ncol)))
+        k = len(sv)
+        M = (u[:, :k] * np.asarray(sv)[None, :]) @ v[:, :k].conj().T
+        pos = 0
+        for qa, qs in keys:
+            n = a_l[qa] * s_l[qs]
+            t.push_back_ch((qa, qs, q), M[pos:pos + n].reshape(a_l[qa], s_l[qs], ncol))
+            pos += n
+    return t
+
+
+@pytest.mark.parametrize("l2r", [True, False])
+def test_svd_isometric_route_tail_refinement(kv, l2r):
+    """Sectors of 100 and 120 singular values spanning 1 .. 1e-14 with none in
+    [0.5e-10, 2e-10] around the 1e-10 cutoff: the isometric-side Gram route must keep
+    exactly the values above the cutoff (second pass over the tail), give the oracle's
+    truncated product and an isometry orthonormal to machine precision; a maxdim-limited
+    cut must match too."""
+    print(f"case svd iso l2r={l2r}", flush=True)
+    rng = np.random.default_rng(12)
+
+    def spec(n, c):
+        v = c * 10.0 ** (-14.0 * np.arange(n) / (n - 1))
+        band = (v > 0.5e-10) & (v < 2e-10)
+        return np.where(band & (v >= 1e-10), 2.5e-10, np.where(band, 0.4e-10, v))
+
+    x = _spectrum_tensor(rng, {0: spec(100, 1.0), 1: spec(120, 0.5)})
+    for cutoff, maxdim in [(1e-10, 1000), (1e-10, 90)]:
+        lef, rig, sv = T.svd_impl(x, 2, cutoff, maxdim, l2r, return_svals=True)
+        gl, gr = kv.svd_impl(up(kv, x), 2, cutoff, maxdim, l2r)
+        gl, gr = down(gl), down(gr)
+        assert gl.legs()[2] == lef.legs()[2]
+        rec_o, rec_g = oproduct(lef, rig, 1), oproduct(gl, gr, 1)
+        assert max_diff(rec_g, rec_o) <= 1e-12 * scale(rec_o)
+        iso = gl if l2r else gr
+        gram = oproduct(iso, iso, 2, "C", "N") if l2r else oproduct(iso, iso, 1, "N", "C")
+        for q, blk in gram.blocks.items():
+            np.testing.assert_allclose(blk, np.eye(blk.shape[0]), atol=1e-12)
