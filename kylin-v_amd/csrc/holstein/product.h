@@ -134,7 +134,7 @@ struct DevProduct {
                                 sizeof(T2) == sizeof(double2) ? "f64" : "f32", p.opA, p.opB,
                                 p.flop * 1e-9);
         for (int cfg = 0; cfg < kNumGemmConfigs; ++cfg) {
-            if (!gemm_config_supports(cfg, p.opA, p.opB)) continue;
+            if (!gemm_config_supports(cfg, p.opA, p.opB, sizeof(T2) == sizeof(double2))) continue;
             launch<T2>(dev, cfg, A, B, C, ar, ai);
             ev.start(dev.stream);
             launch<T2>(dev, cfg, A, B, C, ar, ai);

@@ -69,7 +69,8 @@ struct SvdResult {
     int kept = 0;
     int total = 0;                           // singular values before truncation
     std::vector<std::vector<double>> svals;  // per sector pair, kept values
-    int gram_sectors = 0, exact_sectors = 0;
+    int gram_sectors = 0, exact_sectors = 0, refined_sectors = 0;
+    std::string detail;  // per sector: rows x cols, route, s_max, last kept value
 };
 
 struct SvdOptions {
@@ -121,6 +122,7 @@ struct SectorSvd {
     bool refined = false; // isometric route: tail already refined
     bool colside = true;  // Gram of M^H M (else M M^H)
     bool s_absorbed = false;  // the non-isometric factor already carries S
+    double gscale = 1.0;      // G was divided by this (|M|_F^2) before the eigensolver
     DevBuf M, G, S, U, Vh;  // U: nr x ldU, Vh: k x nc
     int ldU = 0;
     std::vector<double> s;  // descending
@@ -179,6 +181,16 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
         gemm_dense(D, 'C', 'N', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nr);
     else  // G = M M^H
         gemm_dense(D, 'N', 'C', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nc);
+    // G is handed to the eigensolver at unit trace (a sector of a normalised state can
+    // have any small norm); the eigenvalues are scaled back after the solve
+    DevBuf sc(D.pool.get(), 1024 * sizeof(double)), nr2(D.pool.get(), sizeof(double));
+    launch_nrm2sq(D.stream, m, X.nr * X.nc, nr2.as<double>(), sc.as<double>());
+    double f2 = 0;
+    KV_HIP(hipMemcpyAsync(&f2, nr2.get(), sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    D.sync();
+    X.gscale = f2 > 0 ? f2 : 1.0;
+    launch_zaxpby(D.stream, X.G.as<double2>(), X.G.as<double2>(), nullptr,
+                  make_double2(1.0 / X.gscale, 0.0), make_double2(0.0, 0.0), (int64_t)n * n);
 }
 
 // Eigen-decomposition of the Gram matrices of one or more sectors of equal size n, in one
@@ -224,7 +236,8 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs) {
     for (int b = 0; b < nb; ++b) {
         KV_REQUIRE(hinfo[b] == 0, ERR_SOLVER, "zheevd did not converge");
         xs[b]->s.resize(n);
-        for (int j = 0; j < n; ++j) xs[b]->s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0));
+        for (int j = 0; j < n; ++j)
+            xs[b]->s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0) * xs[b]->gscale);
     }
 }
 
@@ -304,20 +317,29 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
     T.k = nt;
     T.gram = true;
     T.G = DevBuf(D.pool.get(), (size_t)nt * nt * sizeof(double2));
+    // Mt is scaled to norm ~1 before its Gram matrix is formed, so the eigensolver sees the
+    // same scale in both passes
+    const double ref = std::max(X.s[na], tail * smax);
+    const double2 inv = make_double2(ref > 0 ? 1.0 / ref : 1.0, 0.0);
     if (!X.colside) {  // rows side: Mt = Bt M (nt x nc), Gt = Mt Mt^H
         DevBuf Mt(D.pool.get(), (size_t)nt * X.nc * sizeof(double2));
         gemm_dense(D, 'N', 'N', Bt, n, m, (int)X.nc, Mt.as<double2>(), (int)X.nc, nt, (int)X.nc,
                    (int)X.nr);
+        launch_zaxpby(D.stream, Mt.as<double2>(), Mt.as<double2>(), nullptr, inv,
+                      make_double2(0.0, 0.0), (int64_t)nt * X.nc);
         gemm_dense(D, 'N', 'C', Mt.as<double2>(), (int)X.nc, Mt.as<double2>(), (int)X.nc,
                    T.G.as<double2>(), nt, nt, nt, (int)X.nc);
     } else {  // columns side: Mt = M Bt^H (nr x nt), Gt = Mt^H Mt
         DevBuf Mt(D.pool.get(), (size_t)X.nr * nt * sizeof(double2));
         gemm_dense(D, 'N', 'C', m, (int)X.nc, Bt, n, Mt.as<double2>(), nt, (int)X.nr, nt,
                    (int)X.nc);
+        launch_zaxpby(D.stream, Mt.as<double2>(), Mt.as<double2>(), nullptr, inv,
+                      make_double2(0.0, 0.0), (int64_t)X.nr * nt);
         gemm_dense(D, 'C', 'N', Mt.as<double2>(), nt, Mt.as<double2>(), nt, T.G.as<double2>(),
                    nt, nt, nt, (int)X.nr);
     }
     gram_solve(D, sol, {&T});
+    for (double& v : T.s) v *= (ref > 0 ? ref : 1.0);
     // Bt <- Ybuf Bt (rows y_j^H, ascending) and the tail values replaced
     DevBuf Bn(D.pool.get(), (size_t)nt * n * sizeof(double2));
     gemm_dense(D, 'N', 'N', T.G.as<double2>(), nt, Bt, n, Bn.as<double2>(), n, nt, n, nt);
@@ -462,6 +484,14 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
     }
     for (int p = 0; p < np; ++p) {
         res.total += (int)X[p].s.size();
+        res.refined_sectors += X[p].refined ? 1 : 0;
+        char buf[160];
+        std::snprintf(buf, sizeof buf, " [%lldx%lld %s smax %.2e kept %d last %.2e]",
+                      (long long)X[p].nr, (long long)X[p].nc,
+                      X[p].iso ? (X[p].refined ? "iso2" : "iso") : (X[p].gram ? "gram" : "exact"),
+                      X[p].s.empty() ? 0.0 : X[p].s[0], kept[p],
+                      kept[p] > 0 ? X[p].s[kept[p] - 1] : 0.0);
+        res.detail += buf;
         if (X[p].iso) {
             res.gram_sectors += 1;
             if (kept[p] > 0) iso_vectors(D, X[p], kept[p]);

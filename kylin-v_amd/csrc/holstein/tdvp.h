@@ -105,8 +105,8 @@ class TDVP {
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
-                std::fprintf(stderr, "[kv] sweep L2R bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d\n",
-                             i, i + 1, te, last_iters_, ts, r.kept, r.total);
+                std::fprintf(stderr, "[kv] sweep L2R bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d%s\n",
+                             i, i + 1, te, last_iters_, ts, r.kept, r.total, r.detail.c_str());
             mps_[i] = std::move(r.left);
             if (i < ns - 2) {
                 Clock c3;
@@ -132,8 +132,8 @@ class TDVP {
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
-                std::fprintf(stderr, "[kv] sweep R2L bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d\n",
-                             i - 1, i, te, last_iters_, ts, r.kept, r.total);
+                std::fprintf(stderr, "[kv] sweep R2L bond %d-%d: evl2 %.4f s (%d it) svd %.4f s kept %d/%d%s\n",
+                             i - 1, i, te, last_iters_, ts, r.kept, r.total, r.detail.c_str());
             mps_[i] = std::move(r.right);
             if (i > 1) {
                 Clock c3;

@@ -53,7 +53,7 @@ template <> struct ScalarOf<float2> { using T = float; };
 // a multiple of 8 tiles apart when the tile-column count is) land on one XCD's L2; a
 // remap giving each XCD a contiguous run of tiles measured 20% slower on the H_eff stages.
 template <class T, int OPA, int OPB, int BM, int BN, int BK>
-__global__ __launch_bounds__(256) void zgemm_grouped_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 4096 ? 3 : 1))) void zgemm_grouped_kernel(
     const typename Cx<T>::T2* __restrict__ A, const typename Cx<T>::T2* __restrict__ B,
     typename Cx<T>::T2* __restrict__ C, const GemmTile* __restrict__ tiles,
     const GemmGroup* __restrict__ groups, const GemmSeg* __restrict__ segs, T alpha_re,
@@ -81,74 +81,105 @@ __global__ __launch_bounds__(256) void zgemm_grouped_kernel(
             cim[i][j] = acc_t{0, 0, 0, 0};
         }
 
-    for (int s = 0; s < g.seg_count; ++s) {
+    // K stages run over (segment, k0) pairs.  The next stage's operands are loaded into
+    // registers while the current one is multiplied out of LDS, so global latency hides
+    // behind the MFMAs of the same workgroup.
+    constexpr int NA = (BM * BK) / 256, NB = (BN * BK) / 256;
+    T2 ra[NA], rb[NB];
+    auto load = [&](int s, int k0) {
         const GemmSeg sg = segs[g.seg_begin + s];
         const T2* __restrict__ a = A + sg.a_off;
         const T2* __restrict__ b = B + sg.b_off;
         const int K = sg.k;
-        for (int k0 = 0; k0 < K; k0 += BK) {
 #pragma unroll
-            for (int r = 0; r < (BM * BK) / 256; ++r) {
-                const int e = tid + r * 256;
-                int mm, kk;
-                if (OPA == 0) {
-                    mm = e / BK;
-                    kk = e % BK;
-                } else {
-                    kk = e / BM;
-                    mm = e % BM;
-                }
-                const int gm = m0 + mm, gk = k0 + kk;
-                T2 v;
-                v.x = 0;
-                v.y = 0;
-                if (gm < g.m && gk < K) {
-                    v = (OPA == 0) ? a[(int64_t)gm * sg.lda + gk] : a[(int64_t)gk * sg.lda + gm];
-                    if (OPA == 2) v.y = -v.y;
-                }
-                As[kk][mm] = v;
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + r * 256;
+            int mm, kk;
+            if (OPA == 0) {
+                mm = e / BK;
+                kk = e % BK;
+            } else {
+                kk = e / BM;
+                mm = e % BM;
             }
-#pragma unroll
-            for (int r = 0; r < (BN * BK) / 256; ++r) {
-                const int e = tid + r * 256;
-                int nn, kk;
-                if (OPB == 0) {
-                    kk = e / BN;
-                    nn = e % BN;
-                } else {
-                    nn = e / BK;
-                    kk = e % BK;
-                }
-                const int gn = n0 + nn, gk = k0 + kk;
-                T2 v;
-                v.x = 0;
-                v.y = 0;
-                if (gn < g.n && gk < K) {
-                    v = (OPB == 0) ? b[(int64_t)gk * sg.ldb + gn] : b[(int64_t)gn * sg.ldb + gk];
-                    if (OPB == 2) v.y = -v.y;
-                }
-                Bs[kk][nn] = v;
+            const int gm = m0 + mm, gk = k0 + kk;
+            T2 v;
+            v.x = 0;
+            v.y = 0;
+            if (gm < g.m && gk < K) {
+                v = (OPA == 0) ? a[(int64_t)gm * sg.lda + gk] : a[(int64_t)gk * sg.lda + gm];
+                if (OPA == 2) v.y = -v.y;
             }
-            __syncthreads();
-#pragma unroll
-            for (int kq = 0; kq < BK; kq += 4) {
-                T2 av[SI], bv[SJ];
-#pragma unroll
-                for (int i = 0; i < SI; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
-#pragma unroll
-                for (int j = 0; j < SJ; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
-#pragma unroll
-                for (int i = 0; i < SI; ++i)
-#pragma unroll
-                    for (int j = 0; j < SJ; ++j) {
-                        cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
-                        cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
-                        cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
-                        cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
-                    }
-            }
-            __syncthreads();
+            ra[r] = v;
         }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + r * 256;
+            int nn, kk;
+            if (OPB == 0) {
+                kk = e / BN;
+                nn = e % BN;
+            } else {
+                nn = e / BK;
+                kk = e % BK;
+            }
+            const int gn = n0 + nn, gk = k0 + kk;
+            T2 v;
+            v.x = 0;
+            v.y = 0;
+            if (gn < g.n && gk < K) {
+                v = (OPB == 0) ? b[(int64_t)gk * sg.ldb + gn] : b[(int64_t)gn * sg.ldb + gk];
+                if (OPB == 2) v.y = -v.y;
+            }
+            rb[r] = v;
+        }
+    };
+    int s = 0, k0 = 0;
+    while (s < g.seg_count && segs[g.seg_begin + s].k <= 0) ++s;
+    if (s < g.seg_count) load(s, 0);
+    while (s < g.seg_count) {
+#pragma unroll
+        for (int r = 0; r < NA; ++r) {
+            const int e = tid + r * 256;
+            const int mm = (OPA == 0) ? e / BK : e % BM;
+            const int kk = (OPA == 0) ? e % BK : e / BM;
+            As[kk][mm] = ra[r];
+        }
+#pragma unroll
+        for (int r = 0; r < NB; ++r) {
+            const int e = tid + r * 256;
+            const int nn = (OPB == 0) ? e % BN : e / BK;
+            const int kk = (OPB == 0) ? e / BN : e % BK;
+            Bs[kk][nn] = rb[r];
+        }
+        __syncthreads();
+        int s2 = s, k2 = k0 + BK;
+        if (k2 >= segs[g.seg_begin + s].k) {
+            k2 = 0;
+            ++s2;
+            while (s2 < g.seg_count && segs[g.seg_begin + s2].k <= 0) ++s2;
+        }
+        if (s2 < g.seg_count) load(s2, k2);
+#pragma unroll
+        for (int kq = 0; kq < BK; kq += 4) {
+            T2 av[SI], bv[SJ];
+#pragma unroll
+            for (int i = 0; i < SI; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
+#pragma unroll
+            for (int j = 0; j < SJ; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
+#pragma unroll
+            for (int i = 0; i < SI; ++i)
+#pragma unroll
+                for (int j = 0; j < SJ; ++j) {
+                    cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
+                    cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
+                    cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
+                    cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
+                }
+        }
+        __syncthreads();
+        s = s2;
+        k0 = k2;
     }
     // epilogue: MFMA output layout per type (Cx<T>::out_row)
 #pragma unroll
@@ -203,7 +234,9 @@ void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2*
     KV_GEMM_HOT_OPS(1, 64, 64, 16)
     KV_GEMM_HOT_OPS(2, 128, 64, 8)
     KV_GEMM_HOT_OPS(3, 64, 128, 8)
-    KV_GEMM_HOT_OPS(4, 128, 128, 8)
+    if constexpr (sizeof(T) == sizeof(float)) {  // the FP64 128x128 tile spills registers
+        KV_GEMM_HOT_OPS(4, 128, 128, 8)
+    }
     KV_GEMM_HOT_OPS(5, 128, 64, 16)
     KV_GEMM_HOT_OPS(6, 64, 128, 16)
 #undef KV_GEMM_HOT_OPS
@@ -215,8 +248,9 @@ void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2*
     throw Error(ERR_ARG, "launch_gemm: bad op codes");
 }
 
-bool gemm_config_supports(int cfg, int opA, int opB) {
+bool gemm_config_supports(int cfg, int opA, int opB, bool fp64) {
     if (cfg == 0) return opA >= 0 && opA <= 2 && opB >= 0 && opB <= 2;
+    if (cfg == 4 && fp64) return false;
     return (opA == 0 && opB == 0) || (opA == 0 && opB == 1) || (opA == 2 && opB == 0) ||
            (opA == 0 && opB == 2);
 }

@@ -441,7 +441,8 @@ def test_svd_isometric_route_tail_refinement(kv, l2r):
         gl, gr = down(gl), down(gr)
         assert gl.legs()[2] == lef.legs()[2]
         rec_o, rec_g = oproduct(lef, rig, 1), oproduct(gl, gr, 1)
-        assert max_diff(rec_g, rec_o) <= 1e-12 * scale(rec_o)
+        # one Gram pass alone leaves ~1e-9 here (the tail vectors are mixed at eps/s^2)
+        assert max_diff(rec_g, rec_o) <= 1e-10 * scale(rec_o)
         iso = gl if l2r else gr
         gram = oproduct(iso, iso, 2, "C", "N") if l2r else oproduct(iso, iso, 1, "N", "C")
         for q, blk in gram.blocks.items():
