@@ -23,6 +23,12 @@ sys.path.insert(0, ROOT)
 
 METRIC = "TDVP steps/s + H_eff TFLOP/s vs FP64 GPU path at max|ΔC(t)|≤1e-6"
 PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s (AMD published)
+# HBM bytes per launch of zgemm_grouped_kernel<float> in this workload's mixed path, from
+# two rocprofv3 --pmc passes over `bench.py --steps 1 --no-cpu --physical-steps 0`
+# (FETCH_SIZE, WRITE_SIZE; mean over 1096 dispatches), each scaled by what the counter
+# reads for f64_to_f32_kernel moving a known 16+8 bytes per element (FETCH_SIZE x2.0,
+# WRITE_SIZE x1.0).  Summary and commands: profiles/c2_merged_D256_gemm_f32_hbm_traffic.md
+GEMM_F32_TRAFFIC_BYTES = 0.390e9
 N, L, OMEGA, J, DT, D, CUTOFF = 32, 16, 0.5, 1.0, 0.02, 256, 1e-10
 
 
@@ -129,6 +135,7 @@ def summarize(r, precision, steps):
                 heff_gemm_launches=int(st["heff_gemm_launches"]),
                 heff_gemm_ms_per_launch=1e3 * st["heff_gemm_gpu_s"] / max(1, st["heff_gemm_launches"]),
                 heff_gemm_gflop_per_launch=st["heff_gemm_flop"] / max(1, st["heff_gemm_launches"]) / 1e9,
+                heff_gemm_gb_per_launch=st["heff_gemm_bytes"] / max(1, st["heff_gemm_launches"]) / 1e9,
                 module_s={k: round(st[k], 3) for k in ("Evl-2", "Evl-1", "SVD", "Update")},
                 matvecs=int(st["matvecs"]), max_bond=max(r["bonds"]))
 
@@ -190,7 +197,10 @@ def main():
         "C_t_fp64_physical_start": ct_phys,
         "roofline": {"kernel": "zgemm_grouped_kernel<float> (H_eff matvec stages)", "bound": "mfma",
                      "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": ach / PEAK["fp32"], "traffic": None,
+                     "frac": ach / PEAK["fp32"], "traffic": GEMM_F32_TRAFFIC_BYTES,
+                     "algorithmic_bytes_per_launch": s32["heff_gemm_gb_per_launch"] * 1e9,
+                     "flop_per_launch": s32["heff_gemm_gflop_per_launch"] * 1e9,
+                     "ms_per_launch": s32["heff_gemm_ms_per_launch"],
                      "fp64_kernel": {"achieved": s64["heff_gemm_tflops"], "peak": PEAK["fp64"],
                                      "frac": s64["heff_gemm_tflops"] / PEAK["fp64"]}},
         "cpu_baseline": cpu,

@@ -182,7 +182,8 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
  *   0 Evl-2  1 Evl-1  2 Evl-0  3 SVD  4 Update  5 Sweep  (wall seconds)
  *   6 heff_gpu_s  7 heff_flop  8 env_gpu_s  9 env_flop  10 matvecs
  *   11 heff_gemm_gpu_s  12 heff_gemm_flop  13 heff_gemm_launches
- *   14 heff_wfold_gpu_s 15 heff_wfold_flop 16 heff_wfold_launches */
+ *   14 heff_wfold_gpu_s 15 heff_wfold_flop 16 heff_wfold_launches
+ *   17 heff_gemm_bytes (algorithmic: each GEMM's operands and result counted once) */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

@@ -43,6 +43,7 @@ struct ProductPlan {
     std::vector<GemmSeg> segs;
     double flop = 0;     // 8*m*n*k summed over block pairs
     double bytes = 0;    // operand + result bytes per complex element size 1
+    double unique = 0;   // elements of A, B and the result, each counted once
 };
 
 // product<N1,N2,nc>(m1, m2, T1, T2) (holstein/product.h:61-201).
@@ -177,6 +178,7 @@ inline ProductPlan plan_product(const Layout& A, const Layout& B, int nc, char o
         P.bytes += (double)(m * n);
         P.groups.push_back(g);
     }
+    P.unique = (double)A.total + (double)B.total;
     // zero groups for target blocks that receive nothing
     for (size_t ob = 0; ob < out.blocks.size(); ++ob) {
         if (covered[ob]) continue;
@@ -192,6 +194,7 @@ inline ProductPlan plan_product(const Layout& A, const Layout& B, int nc, char o
     }
     P.tiles = cut_tiles(P.groups, 0);
     P.out = std::move(out);
+    P.unique += (double)P.out.total;
     return P;
 }
 

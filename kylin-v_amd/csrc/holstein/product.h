@@ -22,7 +22,8 @@ struct EventLog {
     struct Rec {
         hipEvent_t a, b;
         double flop;
-        int kind;  // 0 gemm, 1 wfold
+        int kind;      // 0 gemm, 1 wfold
+        double bytes;  // algorithmic bytes: operands and result once each
     };
     std::vector<Rec> pending;
     std::vector<hipEvent_t> spare;
@@ -44,6 +45,7 @@ struct EventLog {
             if (r.kind == 0) {
                 st.gemm_ms += ms;
                 st.gemm_flop += r.flop;
+                st.gemm_bytes += r.bytes;
                 st.gemm_launches += 1;
             } else {
                 st.wfold_ms += ms;
@@ -176,7 +178,7 @@ struct DevProduct {
         }
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
-            log->pending.push_back({a, b, p.flop, 0});
+            log->pending.push_back({a, b, p.flop, 0, p.unique * (double)sizeof(T2)});
         }
     }
 };
@@ -201,7 +203,7 @@ struct DevWFold {
                          terms.as<WTerm>(), p.maxP);
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
-            log->pending.push_back({a, b, p.flop, 1});
+            log->pending.push_back({a, b, p.flop, 1, 0.0});
         }
     }
 };

@@ -2,16 +2,17 @@
 // Follows svd_impl<R1,R2> (holstein/svd.h:296-440); the sector structure and the
 // truncation rule are in svd_rule.h.
 //
-// Per charge sector two FP64 routes:
+// Per charge sector three FP64 routes (sectors whose smaller side is below iso_min, and
+// zero-keeping truncations that cannot take a Gram route, use the first):
 //  * exact: rocSOLVER one-sided Jacobi (zgesvdj), or zgesvd when cutoff < 0 keeps zero
 //    singular values (their vectors must still be an orthonormal completion);
-//  * Gram (sectors whose smaller side exceeds GRAM_MIN): G = M^H M (or M M^H) on the
-//    MFMA GEMM, Hermitian eigen-decomposition (zheevd), then only the kept singular
-//    vectors by one more GEMM.  zheevd of a 2048-side Gram matrix takes ~0.17 s on
-//    MI355X against ~2 s for the Jacobi SVD of the 2048 x 4096 sector.  Eigenvalues of G
-//    carry absolute error ~ n eps lambda_max, so a sector whose smallest kept singular
-//    value falls below guard * s_max is recomputed on the exact route.
-//  * isometric-side Gram (sectors whose isometric side is at most twice the other):
+//  * smaller-side Gram (when the isometry is the larger side): G on the smaller side by
+//    the MFMA GEMM, Hermitian eigen-decomposition (zheevd), the isometry as M V / s by
+//    one more GEMM.  One batched zheevd of the two 2048-side Gram matrices of a C2 bond
+//    takes ~0.09 s on MI355X against ~2 s per sector for the Jacobi SVD.  M V / s loses orthonormality like
+//    eps (s_max / s)^2, so a sector that keeps values below guard * s_max is redone on the
+//    isometric side;
+//  * isometric-side Gram (when the isometry is the smaller or equal side):
 //    G = M M^H when the left factor is the isometry (S goes right), M^H M when the right
 //    one is; the eigenvectors of G are the isometry itself, orthonormal to machine
 //    precision, and the other factor is one GEMM (U^H M or M V) that already carries S,
@@ -74,16 +75,14 @@ struct SvdResult {
 };
 
 struct SvdOptions {
-    int gram_min = 512;     // smaller side above which the Gram route is used
-    double guard = 1e-5;    // Gram route only if s_kept_min >= guard * s_max
-    int iso_min = 64;       // smaller side from which the isometric-side Gram route is used
+    double guard = 1e-4;    // smaller-side Gram vectors only if s_kept_min >= guard * s_max
+    int iso_min = 64;       // smaller side from which the Gram routes are used
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
     static SvdOptions from_env() {
         SvdOptions o;
         if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
-        if (const char* m = std::getenv("KV_GRAM_MIN")) o.gram_min = std::atoi(m);
         if (const char* m = std::getenv("KV_ISO_MIN")) o.iso_min = std::atoi(m);
         if (const char* t = std::getenv("KV_ISO_TAIL")) o.tail = std::atof(t);
         return o;
@@ -435,14 +434,11 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         }
         const int64_t iso_side = left2right ? rs.len : cs.len;
         const int64_t oth_side = left2right ? cs.len : rs.len;
-        const bool iso = !opt.force_exact && std::min(rs.len, cs.len) >= opt.iso_min &&
-                         iso_side <= 2 * oth_side;
-        const bool gram = !opt.force_exact && cutoff >= 0.0 &&
-                          std::min(rs.len, cs.len) > opt.gram_min;
-        if (iso)
-            gram_prepare(D, X[p], true, !left2right);
-        else if (gram)
-            gram_prepare(D, X[p]);
+        const bool big = !opt.force_exact && std::min(rs.len, cs.len) >= opt.iso_min;
+        if (big && iso_side <= oth_side)
+            gram_prepare(D, X[p], true, !left2right);  // the isometry is the smaller side
+        else if (big && cutoff >= 0.0)
+            gram_prepare(D, X[p]);  // Gram on the smaller side; guarded below
         else
             exact_sector(D, sol, X[p], cutoff < 0.0);
     }
@@ -459,7 +455,9 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
     // phase 2: global truncation; Gram sectors that would keep values too small for the
     // Gram eigenvalues' absolute accuracy move to the exact route and the rule reruns
     std::vector<int> kept;
-    for (int pass = 0; pass < 3; ++pass) {
+    // each sector changes route at most once and is refined at most once, so the rule
+    // settles within four passes
+    for (int pass = 0; pass < 4; ++pass) {
         std::vector<std::vector<double>> sv(np);
         for (int p = 0; p < np; ++p) sv[p] = X[p].s;
         kept = truncation_counts(sv, cutoff, maxdim);
@@ -476,7 +474,11 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
                     redo = true;
                 }
             } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
-                exact_sector(D, sol, X[p], false);
+                // U = M V / s would lose orthonormality at these values: redo the sector
+                // on its isometric (larger) side
+                gram_prepare(D, X[p], true, !left2right);
+                gram_solve(D, sol, {&X[p]});
+                X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
                 redo = true;
             }
         }

@@ -151,7 +151,7 @@ class DevBuf {
 
 // Counters accumulated by the contraction executors.
 struct KernelStats {
-    double gemm_flop = 0, gemm_ms = 0;
+    double gemm_flop = 0, gemm_ms = 0, gemm_bytes = 0;
     double wfold_flop = 0, wfold_ms = 0;
     long gemm_launches = 0, wfold_launches = 0;
 };
