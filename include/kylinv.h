@@ -171,6 +171,11 @@ int kv_tdvp_destroy(kv_tdvp* s);
  * (holstein/tdvp.h:78-141). */
 int kv_tdvp_two_site(kv_tdvp* s);
 int kv_tdvp_one_site(kv_tdvp* s);
+/* One global Krylov step of the whole MPS: exp(-i dt H) by a Lanczos recurrence over MPS
+ * (global_main, holstein/krylov.h:206-284, with H scaled by dt as holstein/tdvp.h:318-327),
+ * truncating with the config's cutoff and maxdim, then rebuilding the environments.
+ * iterations (may be NULL) receives the number of Krylov vectors. */
+int kv_tdvp_global_krylov(kv_tdvp* s, int* iterations);
 /* A copy of MPS site i (caller destroys it). */
 int kv_tdvp_get_site(kv_tdvp* s, int i, kv_tensor** out);
 /* Bond dimensions (nsites-1 ints), cf. State::max_bond (holstein/state.h:187-200). */

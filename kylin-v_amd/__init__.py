@@ -88,6 +88,7 @@ SIGNATURES = {
     "kv_tdvp_destroy": (_I, [_P]),
     "kv_tdvp_two_site": (_I, [_P]),
     "kv_tdvp_one_site": (_I, [_P]),
+    "kv_tdvp_global_krylov": (_I, [_P, _IP]),
     "kv_tdvp_get_site": (_I, [_P, _I, _PP]),
     "kv_tdvp_bond_dims": (_I, [_P, _IP]),
     "kv_tdvp_stats": (_I, [_P, _DP, _I]),
@@ -473,13 +474,26 @@ class TDVP:
     def reset_stats(self):
         self.ctx.check(self.ctx.lib.kv_tdvp_reset_stats(self.h))
 
+    def global_krylov(self):
+        """One global Krylov step (global_main, holstein/krylov.h:206-284); returns the
+        number of Krylov vectors."""
+        n = ctypes.c_int()
+        self.ctx.check(self.ctx.lib.kv_tdvp_global_krylov(self.h, ctypes.byref(n)))
+        return n.value
+
     def total_start(self, stage, on_step=None):
-        """Stage list [nK, n2, nSA, n1] (parse_stage, holstein/tdvp.h:19-32).  Only the
-        2TDVP and 1TDVP stages exist here; nK and nSA must be 0."""
+        """Stage list [nK, n2, nSA, n1] (parse_stage, holstein/tdvp.h:19-32, stages as
+        total_start, holstein/tdvp.h:318-383).  The SA-1TDVP stage is not part of this
+        engine; nSA must be 0."""
         nK, n2, nSA, n1 = stage
-        if nK or nSA:
-            raise KVError(-1, "global-Krylov and SA-1TDVP stages are not part of this engine")
+        if nSA:
+            raise KVError(-1, "the SA-1TDVP stage is not part of this engine")
         k = 0
+        for _ in range(nK):
+            self.global_krylov()
+            if on_step:
+                on_step(k, self)
+            k += 1
         for _ in range(n2):
             self.two_site()
             if on_step:

@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "holstein/io.h"
+#include "holstein/krylov.h"
 #include "holstein/tdvp.h"
 
 struct kv_ctx {
@@ -404,6 +405,14 @@ int kv_tdvp_get_site(kv_tdvp* s, int i, kv_tensor** out) {
         auto t = s->s->site(i).clone();
         s->ctx->dev->sync();
         *out = wrap(s->ctx, std::move(t));
+    });
+}
+
+int kv_tdvp_global_krylov(kv_tdvp* s, int* iterations) {
+    if (!s || !s->s) return KV_ERR_ARG;
+    return guard(s->ctx, [&] {
+        kv::GlobalKrylovExit ex = s->s->global_krylov();
+        if (iterations) *iterations = ex.iterations;
     });
 }
 

@@ -155,10 +155,46 @@ struct DevProduct {
         dev.gemm_tuned[key] = best;
         return best;
     }
+    void launch_split(Device& dev, int cfg, const float2* A, const float2* B, float2* C, double ar,
+                      double ai) {
+        ensure_tiles(dev, cfg);
+        launch_gemm_bf16x3(dev.stream, p.opA, p.opB, A, B, C, tiles[cfg].as<GemmTile>(),
+                           ntiles[cfg], groups.as<GemmGroup>(), segs.as<GemmSeg>(), ar, ai, cfg);
+    }
+    // Split-BF16 tile (64x64 or 128x128), timed once per shape class like choose().
+    int choose_split(Device& dev, const float2* A, const float2* B, float2* C, double ar, double ai) {
+        if (const char* t = std::getenv("KV_SPLIT_TILE")) return std::atoi(t) == 128 ? 4 : 0;
+        if (!dev.autotune || p.flop < 1e8) return 0;
+        const uint64_t key = gemm_shape_key(p, 2);
+        auto it = dev.gemm_tuned.find(key);
+        if (it != dev.gemm_tuned.end()) return it->second;
+        EventTimer ev;
+        int best = 0;
+        float best_ms = 1e30f;
+        for (int cfg : {0, 4}) {
+            launch_split(dev, cfg, A, B, C, ar, ai);
+            ev.start(dev.stream);
+            launch_split(dev, cfg, A, B, C, ar, ai);
+            ev.stop(dev.stream);
+            const float ms = ev.ms();
+            if (ms < best_ms) {
+                best_ms = ms;
+                best = cfg;
+            }
+        }
+        dev.gemm_tuned[key] = best;
+        return best;
+    }
     template <class T2>
     void run(Device& dev, const T2* A, const T2* B, T2* C, double alpha_re = 1.0,
              double alpha_im = 0.0, EventLog* log = nullptr, bool split = false) {
-        const int cfg = split ? 0 : choose<T2>(dev, A, B, C, alpha_re, alpha_im);
+        int cfg = 0;
+        if constexpr (std::is_same<T2, float2>::value) {
+            cfg = split ? choose_split(dev, A, B, C, alpha_re, alpha_im)
+                        : choose<T2>(dev, A, B, C, alpha_re, alpha_im);
+        } else {
+            cfg = choose<T2>(dev, A, B, C, alpha_re, alpha_im);
+        }
         hipEvent_t a = nullptr, b = nullptr;
         if (log) {
             a = log->get();
@@ -167,9 +203,7 @@ struct DevProduct {
         }
         if constexpr (std::is_same<T2, float2>::value) {
             if (split)
-                launch_gemm_bf16x3(dev.stream, p.opA, p.opB, A, B, C, tiles[0].as<GemmTile>(),
-                                   ntiles[0], groups.as<GemmGroup>(), segs.as<GemmSeg>(),
-                                   alpha_re, alpha_im);
+                launch_split(dev, cfg, A, B, C, alpha_re, alpha_im);
             else
                 launch<T2>(dev, cfg, A, B, C, alpha_re, alpha_im);
         } else {

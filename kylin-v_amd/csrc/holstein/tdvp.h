@@ -28,7 +28,10 @@ struct TdvpConfig {
 // Wall-clock module timers with the reference's Timer names (holstein/tdvp.h:85-214).
 struct TdvpTimers {
     double evl2 = 0, evl1 = 0, evl0 = 0, svd = 0, update = 0, sweep = 0;
+    double global = 0;  // global Krylov steps ("Krylov-Iter" + "Krylov-Diag")
 };
+
+struct GlobalKrylovExit;
 
 inline std::unique_ptr<Tensor> left_boundary(const std::shared_ptr<Device>& dev) {
     Layout l;
@@ -82,6 +85,13 @@ class TDVP {
         for (auto& w : mpo_) w->host_data();
         const char* v = std::getenv("KV_VERBOSE");
         verbose_ = v && v[0] == '1';
+        rebuild_environments();
+    }
+
+    // Boundary environments and the right stack (holstein/tdvp.h:300-316).
+    void rebuild_environments() {
+        envL_.clear();
+        envR_.clear();
         envL_.push_back(left_boundary(dev_));
         const Layout& last = mps_.back()->lay;
         KV_REQUIRE(!last.blocks.empty(), ERR_ARG, "empty MPS site");
@@ -89,6 +99,9 @@ class TDVP {
         for (int i = (int)mps_.size() - 1; i > 0; --i)
             envR_.push_back(update(*envR_.back(), *mps_[i], *mpo_[i], false));
     }
+
+    // One global Krylov step (defined in krylov.h).
+    GlobalKrylovExit global_krylov();
 
     // One time step (holstein/tdvp.h:144-227).
     void two_site() {

@@ -7,8 +7,8 @@
 // After every sweep the MPS is written to State/T-<sweep>.mps in the reference's binary
 // format (holstein/tdvp.h:325-338) and "Max bond dimension = D on bond i-j" is printed
 // (holstein/state.h:187-200).  Extra flags: -p fp64|fp32|bf16x3 (H_eff contraction precision),
-// -g device.  The global-Krylov (nK) and SA-1TDVP (nSA) stages are not part of this engine
-// and are rejected.
+// -g device.  The nK stage runs global Krylov steps (holstein/krylov.h:206-284) before
+// the sweeps, as holstein/tdvp.h:318-327; the SA-1TDVP (nSA) stage is rejected.
 #include <getopt.h>
 #include <sys/stat.h>
 
@@ -83,8 +83,8 @@ int main(int argc, char** argv) {
         std::fprintf(stderr, "kln-tdvp: bad stage string %s\n", stage.c_str());
         return 1;
     }
-    if (st[0] != 0 || st[2] != 0) {
-        std::fprintf(stderr, "kln-tdvp: global-Krylov and SA-1TDVP stages are not supported\n");
+    if (st[2] != 0) {
+        std::fprintf(stderr, "kln-tdvp: the SA-1TDVP stage is not supported\n");
         return 1;
     }
     kv_ctx* ctx = nullptr;
@@ -109,12 +109,14 @@ int main(int argc, char** argv) {
     mkdir("State", 0755);
     std::vector<int> bonds(ns > 1 ? ns - 1 : 1);
     int swp = 0;
-    for (int stage_i = 0; stage_i < 2; ++stage_i) {
-        const int n = stage_i == 0 ? st[1] : st[3];
-        std::printf("\n ---------------- Start %s stage ----------------\n", stage_i == 0 ? "2TDVP" : "1TDVP");
+    const char* stage_name[3] = {"Krylov", "2TDVP", "1TDVP"};
+    for (int stage_i = 0; stage_i < 3; ++stage_i) {
+        const int n = stage_i == 0 ? st[0] : stage_i == 1 ? st[1] : st[3];
+        std::printf("\n ---------------- Start %s stage ----------------\n", stage_name[stage_i]);
         for (int k = 0; k < n; ++k, ++swp) {
-            std::printf("\nSweep %d:\n", swp + 1);
-            rc = stage_i == 0 ? kv_tdvp_two_site(drv) : kv_tdvp_one_site(drv);
+            if (stage_i > 0) std::printf("\nSweep %d:\n", swp + 1);
+            rc = stage_i == 0 ? kv_tdvp_global_krylov(drv, nullptr)
+                 : stage_i == 1 ? kv_tdvp_two_site(drv) : kv_tdvp_one_site(drv);
             if (rc) die(ctx, rc, "sweep");
             kv_tdvp_bond_dims(drv, bonds.data());
             int mb = 0, mbp = 0;

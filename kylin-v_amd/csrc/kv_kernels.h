@@ -16,10 +16,11 @@ void launch_gemm(hipStream_t st, int opA, int opB, const T2* A, const T2* B, T2*
                  const GemmSeg* segs, double alpha_re, double alpha_im, int cfg = 0);
 // Tile configs built per op pair and precision (every config is free of register spills).
 bool gemm_config_supports(int cfg, int opA, int opB, bool fp64);
-// Split-BF16 variant on the 64x64 tile list (config 0): FP32 complex operands and output.
+// Split-BF16 variant on the tile list of config 0 (64x64) or 4 (128x128): FP32 complex
+// operands and output.
 void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const float2* B,
                         float2* C, const GemmTile* tiles, int ntiles, const GemmGroup* groups,
-                        const GemmSeg* segs, double alpha_re, double alpha_im);
+                        const GemmSeg* segs, double alpha_re, double alpha_im, int cfg);
 
 template <class T2>
 void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,

@@ -268,9 +268,8 @@ template void launch_gemm<float2>(hipStream_t, int, int, const float2*, const fl
 // Every FP32 operand x is split into x_hi = bf16(x) and x_lo = bf16(x - x_hi) while it is
 // staged into LDS; a real product is x_hi*y_hi + x_hi*y_lo + x_lo*y_hi on
 // v_mfma_f32_16x16x32_bf16 (the lo*lo term, ~2^-16 relative, is dropped).  A complex
-// product is four real products, i.e. 12 MFMAs per 16x16x32 complex step: re.re and im.im
-// accumulate separately and are subtracted in the epilogue.  Same tiles, groups and K
-// segments as zgemm_grouped_kernel with the 64x64 tile.
+// product is four real products, i.e. 12 MFMAs per 16x16x32 complex step.  Same tiles,
+// groups and K segments as zgemm_grouped_kernel.
 // LDS holds each operand as four planes (re_hi, re_lo, im_hi, im_lo), K contiguous per row
 // of A and per column of B, so a lane's eight K values are one 16-byte LDS read:
 // v_mfma_f32_16x16x32_bf16 lane l takes A[l%16][8(l/16)..8(l/16)+7] and
@@ -284,12 +283,22 @@ __device__ inline void split2(float x0, float x1, bf16x2& hi, bf16x2& lo) {
     lo = bf16x2{(__bf16)(x0 - (float)h0), (__bf16)(x1 - (float)h1)};
 }
 
-template <int OPA, int OPB, int BK>
-__global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
+// NT threads = 64 * (BM / (16 SI)) * (BN / (16 SJ)); each wave owns SI x SJ sub-tiles of
+// 16x16 outputs.  The im.im product is accumulated with the B imaginary planes sign-
+// flipped (exact in bf16), so each sub-tile keeps two accumulators (re, im).
+typedef unsigned short u16x8 __attribute__((ext_vector_type(8)));
+
+__device__ inline bf16x8 bf16_neg(bf16x8 v) {
+    return __builtin_bit_cast(bf16x8, __builtin_bit_cast(u16x8, v) ^ (u16x8)(unsigned short)0x8000);
+}
+
+template <int OPA, int OPB, int BM, int BN, int SI, int SJ>
+__global__ __launch_bounds__(64 * (BM / (16 * SI)) * (BN / (16 * SJ))) void zgemm_bf16x3_kernel(
     const float2* __restrict__ A, const float2* __restrict__ B, float2* __restrict__ C,
     const GemmTile* __restrict__ tiles, const GemmGroup* __restrict__ groups,
     const GemmSeg* __restrict__ segs, float alpha_re, float alpha_im) {
-    constexpr int BM = 64, BN = 64, LDK = BK + 8;  // row pitch stays a multiple of 16 bytes
+    constexpr int BK = 32, LDK = BK + 8;  // row pitch stays a multiple of 16 bytes
+    constexpr int WN = BN / (16 * SJ), NT = 64 * (BM / (16 * SI)) * WN;
     __shared__ __attribute__((aligned(16))) __bf16 As[4][BM][LDK];
     __shared__ __attribute__((aligned(16))) __bf16 Bs[4][BN][LDK];
 
@@ -298,16 +307,15 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
     const int m0 = tile.tm * BM, n0 = tile.tn * BN;
     const int tid = threadIdx.x;
     const int lane = tid & 63, wave = tid >> 6;
-    const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+    const int wm = (wave / WN) * (16 * SI), wn = (wave % WN) * (16 * SJ);
     const int lr = lane & 15, lk = lane >> 4;
 
-    f32x4 crr[2][2], cii[2][2], cim[2][2];
+    f32x4 cre[SI][SJ], cim[SI][SJ];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j) {
-            crr[i][j] = f32x4{0, 0, 0, 0};
-            cii[i][j] = f32x4{0, 0, 0, 0};
+        for (int j = 0; j < SJ; ++j) {
+            cre[i][j] = f32x4{0, 0, 0, 0};
             cim[i][j] = f32x4{0, 0, 0, 0};
         }
 
@@ -319,8 +327,8 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
         for (int k0 = 0; k0 < K; k0 += BK) {
             // stage A: pairs of consecutive K per thread
 #pragma unroll
-            for (int r = 0; r < (BM * BK / 2) / 256; ++r) {
-                const int e = tid + r * 256;
+            for (int r = 0; r < (BM * BK / 2) / NT; ++r) {
+                const int e = tid + r * NT;
                 int mm, kk;
                 if (OPA == 0) {
                     mm = e / (BK / 2);
@@ -355,8 +363,8 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
             }
             // stage B
 #pragma unroll
-            for (int r = 0; r < (BN * BK / 2) / 256; ++r) {
-                const int e = tid + r * 256;
+            for (int r = 0; r < (BN * BK / 2) / NT; ++r) {
+                const int e = tid + r * NT;
                 int nn, kk;
                 if (OPB == 0) {
                     nn = e % BN;
@@ -390,51 +398,53 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
                 *(bf16x2*)&Bs[3][nn][kk] = l;
             }
             __syncthreads();
+            bf16x8 af[SI][4], bf[SJ][4], bn[SJ][2];
 #pragma unroll
-            for (int kq = 0; kq < BK; kq += 32) {
-                bf16x8 af[2][4], bf[2][4];
+            for (int i = 0; i < SI; ++i)
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
+                for (int p = 0; p < 4; ++p)
+                    af[i][p] = *(const bf16x8*)&As[p][wm + i * 16 + lr][8 * lk];
 #pragma unroll
-                    for (int p = 0; p < 4; ++p) {
-                        af[i][p] = *(const bf16x8*)&As[p][wm + i * 16 + lr][kq + 8 * lk];
-                        bf[i][p] = *(const bf16x8*)&Bs[p][wn + i * 16 + lr][kq + 8 * lk];
-                    }
+            for (int j = 0; j < SJ; ++j) {
 #pragma unroll
-                for (int i = 0; i < 2; ++i)
-#pragma unroll
-                    for (int j = 0; j < 2; ++j) {
-                        f32x4 rr = crr[i][j], ii = cii[i][j], im = cim[i][j];
-                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][0], rr, 0, 0, 0);
-                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][1], rr, 0, 0, 0);
-                        rr = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][0], rr, 0, 0, 0);
-                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][2], ii, 0, 0, 0);
-                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][3], ii, 0, 0, 0);
-                        ii = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bf[j][2], ii, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][2], im, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][3], im, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][2], im, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][0], im, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][1], im, 0, 0, 0);
-                        im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bf[j][0], im, 0, 0, 0);
-                        crr[i][j] = rr;
-                        cii[i][j] = ii;
-                        cim[i][j] = im;
-                    }
+                for (int p = 0; p < 4; ++p)
+                    bf[j][p] = *(const bf16x8*)&Bs[p][wn + j * 16 + lr][8 * lk];
+                bn[j][0] = bf16_neg(bf[j][2]);
+                bn[j][1] = bf16_neg(bf[j][3]);
             }
+#pragma unroll
+            for (int i = 0; i < SI; ++i)
+#pragma unroll
+                for (int j = 0; j < SJ; ++j) {
+                    f32x4 re = cre[i][j], im = cim[i][j];
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][0], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][1], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][0], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bn[j][0], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bn[j][1], re, 0, 0, 0);
+                    re = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bn[j][0], re, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][2], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][0], bf[j][3], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][1], bf[j][2], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][0], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][2], bf[j][1], im, 0, 0, 0);
+                    im = __builtin_amdgcn_mfma_f32_16x16x32_bf16(af[i][3], bf[j][0], im, 0, 0, 0);
+                    cre[i][j] = re;
+                    cim[i][j] = im;
+                }
             __syncthreads();
         }
     }
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SI; ++i)
 #pragma unroll
-        for (int j = 0; j < 2; ++j)
+        for (int j = 0; j < SJ; ++j)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
                 const int row = m0 + wm + i * 16 + 4 * lk + v;
                 const int col = n0 + wn + j * 16 + lr;
                 if (row < g.m && col < g.n) {
-                    const float re = crr[i][j][v] - cii[i][j][v], im = cim[i][j][v];
+                    const float re = cre[i][j][v], im = cim[i][j][v];
                     float2 o;
                     o.x = alpha_re * re - alpha_im * im;
                     o.y = alpha_re * im + alpha_im * re;
@@ -443,19 +453,25 @@ __global__ __launch_bounds__(256) void zgemm_bf16x3_kernel(
             }
 }
 
+// cfg 0: 64x64 tile (256 threads); cfg 4: 128x128 tile (512 threads), same tile lists as
+// the FP32 kernel's configs 0 and 4.
 void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const float2* B,
                         float2* C, const GemmTile* tiles, int ntiles, const GemmGroup* groups,
-                        const GemmSeg* segs, double alpha_re, double alpha_im) {
+                        const GemmSeg* segs, double alpha_re, double alpha_im, int cfg) {
     if (ntiles <= 0) return;
-    const dim3 grid(ntiles), block(256);
     const float ar = (float)alpha_re, ai = (float)alpha_im;
 #define KV_BF_CASE(OA, OB)                                                                     \
     if (opA == OA && opB == OB) {                                                              \
-        hipLaunchKernelGGL((zgemm_bf16x3_kernel<OA, OB, 32>), grid, block, 0, st, A, B, C,     \
-                           tiles, groups, segs, ar, ai);                                       \
+        if (cfg == 4)                                                                          \
+            hipLaunchKernelGGL((zgemm_bf16x3_kernel<OA, OB, 128, 128, 4, 2>), dim3(ntiles),     \
+                               dim3(512), 0, st, A, B, C, tiles, groups, segs, ar, ai);        \
+        else                                                                                   \
+            hipLaunchKernelGGL((zgemm_bf16x3_kernel<OA, OB, 64, 64, 2, 2>), dim3(ntiles),       \
+                               dim3(256), 0, st, A, B, C, tiles, groups, segs, ar, ai);        \
         KV_HIP(hipGetLastError());                                                             \
         return;                                                                                \
     }
+    if (cfg != 0 && cfg != 4) throw Error(ERR_ARG, "launch_gemm_bf16x3: tile config 0 or 4");
     KV_BF_CASE(0, 0)
     KV_BF_CASE(0, 1)
     KV_BF_CASE(0, 2)

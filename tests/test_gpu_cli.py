@@ -63,7 +63,42 @@ def test_kln_tdvp_cli_matches_oracle(tmp_path, layout):
         c_cli = T.overlap_states(psi0, o_sites)
         c_ref = T.overlap_states(psi0, ref.mps)
         assert abs(c_cli - c_ref) < 1e-9
-    # bad stage: the global-Krylov stage is rejected with a message, not a crash
-    r2 = subprocess.run([exe, "-w", "Ele-1.mps", "-o", "H_tot", "-s", "[1,3,0,0]", "-v", "0.05"],
+    # the SA-1TDVP stage is rejected with a message, not a crash
+    r2 = subprocess.run([exe, "-w", "Ele-1.mps", "-o", "H_tot", "-s", "[0,3,1,0]", "-v", "0.05"],
                         cwd=tmp_path, capture_output=True, text=True, timeout=60)
     assert r2.returncode == 1 and "not supported" in r2.stderr
+
+
+def test_kln_tdvp_cli_global_krylov_stage(tmp_path):
+    """-s [2,3,0,0] in the projected-purification layout: two global Krylov steps open the
+    exciton-hop sectors (holstein/tdvp.h:318-327), then 2TDVP moves the exciton; C(t) and
+    the populations of every written state against the oracle."""
+    print("case kln-tdvp nK pp", flush=True)
+    kv = GE.load_package()
+    from oracle import krylov as K
+    from oracle.blocksparse import Tensor as OT
+    lat = M.holstein(3, [1.0], np.array([1.0, 0.8, 1.2]), [3], 1.0, "pp")
+    mpo = M.build_mpo(lat)
+    psi0 = M.product_state(lat)
+    write_train(tmp_path / "Ele-1.mps", psi0, 111, 3)
+    write_train(tmp_path / "H_tot", mpo, 112, 4)
+    exe = os.path.join(ROOT, "kylin-v_amd", "kln-tdvp")
+    r = subprocess.run([exe, "-w", "Ele-1.mps", "-o", "H_tot", "-s", "[2,3,0,0]", "-v", "0.05",
+                        "-D", "24", "-t", "1e-10"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=300)
+    print(r.stdout[-600:], r.stderr[-600:], flush=True)
+    assert r.returncode == 0
+    Hc = K.scale_mpo(mpo, 0.05)
+    g1, _ = K.global_main(Hc, psi0, 1e-10, 24)
+    g2, _ = K.global_main(Hc, g1, 1e-10, 24)
+    expected = [g1, g2]
+    eng = T.TDVP(g2, mpo, cutoff=1e-10, maxdim=24, dt=0.05)
+    for _ in range(3):
+        eng.two_site()
+        expected.append([t.copy() for t in eng.mps])
+    for k, want in enumerate(expected):
+        sites = kv.load_mps(str(tmp_path / "State" / f"T-{k}.mps"))
+        got = [OT(*s.to_blocks()) for s in sites]
+        assert abs(T.overlap_states(psi0, got) - T.overlap_states(psi0, want)) < 1e-9
+    n1 = M.density_mpo(lat, 1)
+    assert T.join(n1, got, got).real > 1e-5  # the exciton left its first site

@@ -137,11 +137,15 @@ def test_heff_matches_oracle(kv, layout, precision):
     assert max_diff(got0, y0) <= tol * rounding_scale(T.apply_zero, envl1, nex, envr)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3"])
-def test_heff_large_ragged_sectors(kv, precision):
+@pytest.mark.parametrize("precision", ["fp64", "fp32", "bf16x3", "bf16x3-128"])
+def test_heff_large_ragged_sectors(kv, precision, monkeypatch):
     """One- and two-site H_eff on random block tensors whose sector dims straddle the 64x64
-    output tile and the 32-deep split-BF16 K step (edge tiles, K tails, several K segments)."""
+    output tile and the 32-deep split-BF16 K step (edge tiles, K tails, several K segments);
+    bf16x3-128 forces the split-BF16 kernel's 128x128 tile."""
     print(f"case heff ragged {precision}", flush=True)
+    if precision == "bf16x3-128":
+        monkeypatch.setenv("KV_SPLIT_TILE", "128")
+        precision = "bf16x3"
     rng = np.random.default_rng(5)
     a_l = {0: 70, 1: 33}
     s_l = {0: 5, 1: 3}
