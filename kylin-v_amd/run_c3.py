@@ -24,14 +24,19 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--first", type=int, default=0, help="run chains [first, first+count) of the set")
     ap.add_argument("--count", type=int, default=None)
+    ap.add_argument("--pick", default=None, help="comma-separated chain indices (instead of first/count)")
     args = ap.parse_args()
     import __graft_entry__ as GE
     GE.load_package()
     from kylin_v_amd import batch
 
     specs = batch.c3_chains(args.chains, args.seed)
-    count = len(specs) - args.first if args.count is None else args.count
-    specs = specs[args.first:args.first + count]
+    if args.pick:
+        idx = [int(x) for x in args.pick.split(",")]
+    else:
+        count = len(specs) - args.first if args.count is None else args.count
+        idx = list(range(args.first, args.first + count))
+    specs = [specs[i] for i in idx]
     t0 = time.perf_counter()
 
     import ctypes
@@ -44,14 +49,14 @@ def main():
         return free.value / 2**30
 
     def progress(i, r):
-        print(f"c3: chain {args.first + i} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
+        print(f"c3: chain {idx[i]} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
               f"{r['seconds']:.1f} s, max bond {max(r['bonds'])}, device free {free_gib():.1f} GiB",
               file=sys.stderr, flush=True)
 
     res, wall = batch.run_batch(specs, args.steps, dt=0.05, workers=args.workers,
                                 precision=args.precision, on_progress=progress)
     out = {"workload": "64 Holstein chains, 2TDVP 50 steps dt=0.05 (seeded set)",
-           "chains": args.chains, "first": args.first, "count": len(specs), "steps": args.steps,
+           "chains": args.chains, "picked": idx, "count": len(specs), "steps": args.steps,
            "workers": min(args.workers, batch.MAX_WORKERS),
            "precision": args.precision, "wall_s": wall,
            "chain_steps_per_s": len(specs) * args.steps / wall,
