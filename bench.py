@@ -130,12 +130,20 @@ def summarize(r, precision, steps):
     st = r["stats"]
     heff_tf = st["heff_flop"] / st["heff_gpu_s"] / 1e12 if st["heff_gpu_s"] > 0 else 0.0
     gemm_tf = st["heff_gemm_flop"] / st["heff_gemm_gpu_s"] / 1e12 if st["heff_gemm_gpu_s"] > 0 else 0.0
+    n32 = max(1, st["heff_gemm32_launches"])
+    gemm32 = dict(tflops=st["heff_gemm32_flop"] / st["heff_gemm32_gpu_s"] / 1e12
+                  if st["heff_gemm32_gpu_s"] > 0 else 0.0,
+                  launches=int(st["heff_gemm32_launches"]),
+                  ms_per_launch=1e3 * st["heff_gemm32_gpu_s"] / n32,
+                  gflop_per_launch=st["heff_gemm32_flop"] / n32 / 1e9,
+                  gb_per_launch=st["heff_gemm32_bytes"] / n32 / 1e9)
     return dict(steps_per_s=steps / r["wall"], ms_per_step=1e3 * r["wall"] / steps,
                 heff_tflops=heff_tf, heff_gemm_tflops=gemm_tf,
                 heff_gemm_launches=int(st["heff_gemm_launches"]),
                 heff_gemm_ms_per_launch=1e3 * st["heff_gemm_gpu_s"] / max(1, st["heff_gemm_launches"]),
                 heff_gemm_gflop_per_launch=st["heff_gemm_flop"] / max(1, st["heff_gemm_launches"]) / 1e9,
                 heff_gemm_gb_per_launch=st["heff_gemm_bytes"] / max(1, st["heff_gemm_launches"]) / 1e9,
+                fp32_gemm=gemm32,
                 module_s={k: round(st[k], 3) for k in ("Evl-2", "Evl-1", "SVD", "Update")},
                 matvecs=int(st["matvecs"]), max_bond=max(r["bonds"]))
 
@@ -167,7 +175,8 @@ def main():
     dC_phys, ct_phys = (physical_gate(kv, ch, args.physical_steps) if args.physical_steps > 0
                         else (None, None))
     cpu = None if args.no_cpu else cpu_baseline(kv, ch, psi0)
-    ach = s32["heff_gemm_tflops"]
+    g32 = s32["fp32_gemm"]
+    ach = g32["tflops"]
     out = {
         "metric": METRIC,
         "value": s32["steps_per_s"],
@@ -198,9 +207,9 @@ def main():
         "roofline": {"kernel": "zgemm_grouped_kernel<float> (H_eff matvec stages)", "bound": "mfma",
                      "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s",
                      "frac": ach / PEAK["fp32"], "traffic": GEMM_F32_TRAFFIC_BYTES,
-                     "algorithmic_bytes_per_launch": s32["heff_gemm_gb_per_launch"] * 1e9,
-                     "flop_per_launch": s32["heff_gemm_gflop_per_launch"] * 1e9,
-                     "ms_per_launch": s32["heff_gemm_ms_per_launch"],
+                     "algorithmic_bytes_per_launch": g32["gb_per_launch"] * 1e9,
+                     "flop_per_launch": g32["gflop_per_launch"] * 1e9,
+                     "ms_per_launch": g32["ms_per_launch"], "launches": g32["launches"],
                      "fp64_kernel": {"achieved": s64["heff_gemm_tflops"], "peak": PEAK["fp64"],
                                      "frac": s64["heff_gemm_tflops"] / PEAK["fp64"]}},
         "cpu_baseline": cpu,

@@ -188,7 +188,10 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
  *   6 heff_gpu_s  7 heff_flop  8 env_gpu_s  9 env_flop  10 matvecs
  *   11 heff_gemm_gpu_s  12 heff_gemm_flop  13 heff_gemm_launches
  *   14 heff_wfold_gpu_s 15 heff_wfold_flop 16 heff_wfold_launches
- *   17 heff_gemm_bytes (algorithmic: each GEMM's operands and result counted once) */
+ *   17 heff_gemm_bytes (algorithmic: each GEMM's operands and result counted once)
+ *   18 heff_gemm32_gpu_s 19 heff_gemm32_flop 20 heff_gemm32_launches 21 heff_gemm32_bytes
+ *      (the FP32-operand GEMM launches alone; the mixed path's first matvec per local
+ *      step and every FP64-path launch are FP64) */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

@@ -421,7 +421,8 @@ def save_mpo(path, sites):
 STAT_NAMES = ["Evl-2", "Evl-1", "Evl-0", "SVD", "Update", "Sweep", "heff_gpu_s", "heff_flop",
               "env_gpu_s", "env_flop", "matvecs", "heff_gemm_gpu_s", "heff_gemm_flop",
               "heff_gemm_launches", "heff_wfold_gpu_s", "heff_wfold_flop", "heff_wfold_launches",
-              "heff_gemm_bytes"]
+              "heff_gemm_bytes", "heff_gemm32_gpu_s", "heff_gemm32_flop", "heff_gemm32_launches",
+              "heff_gemm32_bytes"]
 
 
 class TDVP:

@@ -28,7 +28,7 @@ def c3_chains(n=64, seed=3):
     return out
 
 
-MAX_WORKERS = 8
+MAX_WORKERS = 16
 
 
 def chain_cost(s):

@@ -428,14 +428,15 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
     const kv::TdvpTimers& t = s->s->timers();
     const kv::KernelStats& h = s->s->heff_kernels();
     const kv::KernelStats& e = s->s->env_kernels();
-    const double v[18] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
+    const double v[22] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
                           (h.gemm_ms + h.wfold_ms) * 1e-3, h.gemm_flop + h.wfold_flop,
                           (e.gemm_ms + e.wfold_ms) * 1e-3, e.gemm_flop + e.wfold_flop,
                           (double)s->s->lanczos_stats().matvecs,
                           h.gemm_ms * 1e-3, h.gemm_flop, (double)h.gemm_launches,
                           h.wfold_ms * 1e-3, h.wfold_flop, (double)h.wfold_launches,
-                          h.gemm_bytes};
-    for (int i = 0; i < n && i < 18; ++i) out[i] = v[i];
+                          h.gemm_bytes, h.gemm32_ms * 1e-3, h.gemm32_flop,
+                          (double)h.gemm32_launches, h.gemm32_bytes};
+    for (int i = 0; i < n && i < 22; ++i) out[i] = v[i];
     return KV_OK;
 }
 

@@ -22,7 +22,7 @@ struct EventLog {
     struct Rec {
         hipEvent_t a, b;
         double flop;
-        int kind;      // 0 gemm, 1 wfold
+        int kind;      // 0 gemm (FP64 operands), 1 wfold, 2 gemm (FP32 operands)
         double bytes;  // algorithmic bytes: operands and result once each
     };
     std::vector<Rec> pending;
@@ -42,10 +42,16 @@ struct EventLog {
         for (auto& r : pending) {
             float ms = 0;
             KV_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-            if (r.kind == 0) {
+            if (r.kind == 0 || r.kind == 2) {
                 st.gemm_ms += ms;
                 st.gemm_flop += r.flop;
                 st.gemm_bytes += r.bytes;
+                if (r.kind == 2) {
+                    st.gemm32_ms += ms;
+                    st.gemm32_flop += r.flop;
+                    st.gemm32_bytes += r.bytes;
+                    st.gemm32_launches += 1;
+                }
                 st.gemm_launches += 1;
             } else {
                 st.wfold_ms += ms;
@@ -212,7 +218,8 @@ struct DevProduct {
         }
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
-            log->pending.push_back({a, b, p.flop, 0, p.unique * (double)sizeof(T2)});
+            log->pending.push_back({a, b, p.flop, sizeof(T2) == sizeof(float2) ? 2 : 0,
+                                    p.unique * (double)sizeof(T2)});
         }
     }
 };

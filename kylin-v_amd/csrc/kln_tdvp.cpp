@@ -134,8 +134,8 @@ int main(int argc, char** argv) {
             for (auto* t : sites) kv_tensor_destroy(t);
         }
     }
-    double s[18] = {0};
-    kv_tdvp_stats(drv, s, 18);
+    double s[22] = {0};
+    kv_tdvp_stats(drv, s, 22);
     const char* names[6] = {"Evl-2", "Evl-1", "Evl-0", "SVD", "Update", "Sweep"};
     std::printf("\n%-12s %12s\n", "Module", "Wall(s)");
     for (int i = 0; i < 6; ++i) std::printf("%-12s %12.4f\n", names[i], s[i]);

@@ -61,8 +61,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
     using T2 = typename Cx<T>::T2;
     using acc_t = typename Cx<T>::acc;
     constexpr int SI = BM / 32, SJ = BN / 32;  // 16x16 sub-tiles per wave
-    __shared__ T2 As[BK][BM + 1];
-    __shared__ T2 Bs[BK][BN + 1];
+    __shared__ T2 As[2][BK][BM + 1];  // double-buffered: one barrier per K stage
+    __shared__ T2 Bs[2][BK][BN + 1];
 
     const GemmTile tile = tiles[blockIdx.x];
     const GemmGroup g = groups[tile.group];
@@ -134,25 +134,32 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
             rb[r] = v;
         }
     };
-    int s = 0, k0 = 0;
-    while (s < g.seg_count && segs[g.seg_begin + s].k <= 0) ++s;
-    if (s < g.seg_count) load(s, 0);
-    while (s < g.seg_count) {
+    auto store = [&](int buf) {
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + r * 256;
             const int mm = (OPA == 0) ? e / BK : e % BM;
             const int kk = (OPA == 0) ? e % BK : e / BM;
-            As[kk][mm] = ra[r];
+            As[buf][kk][mm] = ra[r];
         }
 #pragma unroll
         for (int r = 0; r < NB; ++r) {
             const int e = tid + r * 256;
             const int nn = (OPB == 0) ? e % BN : e / BK;
             const int kk = (OPB == 0) ? e / BN : e % BK;
-            Bs[kk][nn] = rb[r];
+            Bs[buf][kk][nn] = rb[r];
         }
-        __syncthreads();
+    };
+    int s = 0, k0 = 0, buf = 0;
+    while (s < g.seg_count && segs[g.seg_begin + s].k <= 0) ++s;
+    if (s < g.seg_count) {
+        load(s, 0);
+        store(0);
+    }
+    __syncthreads();
+    // iteration t multiplies out of buffer buf while the operands of stage t+1 go into the
+    // other one; the single barrier ends both (the other buffer was last read in t-1)
+    while (s < g.seg_count) {
         int s2 = s, k2 = k0 + BK;
         if (k2 >= segs[g.seg_begin + s].k) {
             k2 = 0;
@@ -164,9 +171,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
         for (int kq = 0; kq < BK; kq += 4) {
             T2 av[SI], bv[SJ];
 #pragma unroll
-            for (int i = 0; i < SI; ++i) av[i] = As[kq + lk][wm + i * 16 + lr];
+            for (int i = 0; i < SI; ++i) av[i] = As[buf][kq + lk][wm + i * 16 + lr];
 #pragma unroll
-            for (int j = 0; j < SJ; ++j) bv[j] = Bs[kq + lk][wn + j * 16 + lr];
+            for (int j = 0; j < SJ; ++j) bv[j] = Bs[buf][kq + lk][wn + j * 16 + lr];
 #pragma unroll
             for (int i = 0; i < SI; ++i)
 #pragma unroll
@@ -177,7 +184,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
                     cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
                 }
         }
+        if (s2 < g.seg_count) store(buf ^ 1);
         __syncthreads();
+        buf ^= 1;
         s = s2;
         k0 = k2;
     }

@@ -151,9 +151,12 @@ class DevBuf {
 
 // Counters accumulated by the contraction executors.
 struct KernelStats {
-    double gemm_flop = 0, gemm_ms = 0, gemm_bytes = 0;
+    double gemm_flop = 0, gemm_ms = 0, gemm_bytes = 0;  // every GEMM launch
     double wfold_flop = 0, wfold_ms = 0;
     long gemm_launches = 0, wfold_launches = 0;
+    // the FP32-operand launches alone (FP32 and split-BF16 kernels)
+    double gemm32_flop = 0, gemm32_ms = 0, gemm32_bytes = 0;
+    long gemm32_launches = 0;
 };
 
 struct Device {

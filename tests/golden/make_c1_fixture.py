@@ -4,7 +4,7 @@ two-site TDVP dt = 0.05, 40 steps, D <= 32, cutoff 1e-10, Krylov tol 1e-8, 15 it
 
 The oracle (oracle/) is the numpy restatement of the reference's algorithm, pinned by
 tests/test_oracle.py; these files let the GPU test compare the whole 40-step run without
-re-running the oracle on the GPU box.
+re-running the 40-step oracle trajectory inside the GPU test.
 
     python tests/golden/make_c1_fixture.py [merged|pp ...]
 """
