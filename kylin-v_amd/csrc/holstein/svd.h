@@ -7,11 +7,12 @@
 //  * exact: rocSOLVER one-sided Jacobi (zgesvdj), or zgesvd when cutoff < 0 keeps zero
 //    singular values (their vectors must still be an orthonormal completion);
 //  * smaller-side Gram (when the isometry is the larger side): G on the smaller side by
-//    the MFMA GEMM, Hermitian eigen-decomposition (zheevd), the isometry as M V / s by
-//    one more GEMM.  One batched zheevd of the two 2048-side Gram matrices of a C2 bond
-//    takes ~0.09 s on MI355X against ~2 s per sector for the Jacobi SVD.  M V / s loses orthonormality like
-//    eps (s_max / s)^2, so a sector that keeps values below guard * s_max is redone on the
-//    isometric side;
+//    the MFMA GEMM, Hermitian eigen-decomposition (batched zhetrd, dstedc per sector,
+//    eigenvectors back-transformed only for the kept values), the isometry as M V / s by
+//    one more GEMM.  The eigenproblem of the two 2048-side Gram matrices of a 32-site,
+//    D=256 bond takes ~0.07 s on MI355X against ~2 s per sector for the Jacobi SVD.
+//    M V / s loses orthonormality like eps (s_max / s)^2, so a sector that keeps values
+//    below guard * s_max is redone on the isometric side;
 //  * isometric-side Gram (when the isometry is the smaller or equal side):
 //    G = M M^H when the left factor is the isometry (S goes right), M^H M when the right
 //    one is; the eigenvectors of G are the isometry itself, orthonormal to machine
@@ -51,18 +52,35 @@ inline std::mutex& solver_init_mutex() {
     return m;
 }
 
+// h runs on the context stream; h2 on a second stream of default priority, used to overlap
+// the per-matrix tridiagonal eigensolvers of a batch (fork/join by events).
 struct Solver {
-    rocblas_handle h = nullptr;
+    rocblas_handle h = nullptr, h2 = nullptr;
+    hipStream_t s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     explicit Solver(Device& dev) {
         std::lock_guard<std::mutex> g(solver_init_mutex());
         if (rocblas_create_handle(&h) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_create_handle failed");
         if (rocblas_set_stream(h, dev.stream) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_set_stream failed");
+        KV_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        KV_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        KV_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        if (rocblas_create_handle(&h2) != rocblas_status_success)
+            throw Error(ERR_SOLVER, "rocblas_create_handle failed");
+        if (rocblas_set_stream(h2, s2) != rocblas_status_success)
+            throw Error(ERR_SOLVER, "rocblas_set_stream failed");
     }
     ~Solver() {
         if (h) rocblas_destroy_handle(h);
+        if (h2) rocblas_destroy_handle(h2);
+        if (s2) (void)hipStreamDestroy(s2);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
     }
+    Solver(const Solver&) = delete;
+    Solver& operator=(const Solver&) = delete;
 };
 
 struct SvdResult {
@@ -79,9 +97,11 @@ struct SvdOptions {
     int iso_min = 64;       // smaller side from which the Gram routes are used
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
+    bool eig_heevd = false;  // Gram eigenproblems by zheevd instead of the split solver
     static SvdOptions from_env() {
         SvdOptions o;
         if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
+        if (const char* e = std::getenv("KV_EIG")) o.eig_heevd = std::string(e) == "heevd";
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
         if (const char* m = std::getenv("KV_ISO_MIN")) o.iso_min = std::atoi(m);
         if (const char* t = std::getenv("KV_ISO_TAIL")) o.tail = std::atof(t);
@@ -90,8 +110,10 @@ struct SvdOptions {
 };
 
 // C (m x n, ld ldc) = op(A) op(B) for one dense complex problem on the grouped GEMM.
+// upper_only: only the 64 x 64 tiles on or above the diagonal (Hermitian products whose
+// consumer reads one triangle); the rest of C is left as it was.
 inline void gemm_dense(Device& D, char opA, char opB, const double2* A, int lda, const double2* B,
-                       int ldb, double2* C, int ldc, int m, int n, int k) {
+                       int ldb, double2* C, int ldc, int m, int n, int k, bool upper_only = false) {
     GemmGroup g{};
     g.m = m;
     g.n = n;
@@ -106,7 +128,8 @@ inline void gemm_dense(Device& D, char opA, char opB, const double2* A, int lda,
     std::vector<GemmSeg> ss{sg};
     std::vector<GemmTile> ts;
     for (int a = 0; a < (m + GEMM_BM - 1) / GEMM_BM; ++a)
-        for (int b = 0; b < (n + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{0, a, b, 0});
+        for (int b = 0; b < (n + GEMM_BN - 1) / GEMM_BN; ++b)
+            if (!upper_only || a * GEMM_BM < (b + 1) * GEMM_BN) ts.push_back(GemmTile{0, a, b, 0});
     DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
     launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, C, dt.as<GemmTile>(),
                          (int)ts.size(), dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
@@ -125,6 +148,12 @@ struct SectorSvd {
     DevBuf M, G, S, U, Vh;  // U: nr x ldU, Vh: k x nc
     int ldU = 0;
     std::vector<double> s;  // descending
+    // Gram eigenvectors: rows [ev_lo, k) of the ascending eigenvector matrix, row j - ev_lo of
+    // Ev (ld k) read row-major is v_j^H.  The split solver leaves G holding its Householder
+    // reflectors, tau their scalars and Zt the tridiagonal eigenvectors (real, column-major),
+    // from which gram_eigvecs forms the rows on demand.
+    DevBuf Ev, tau, Zt;
+    int ev_lo = 0;
 };
 
 inline void exact_sector(Device& D, Solver& sol, SectorSvd& X, bool keep_zero) {
@@ -176,10 +205,13 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
     X.gram = true;
     X.G = DevBuf(D.pool.get(), (size_t)n * n * sizeof(double2));
     const double2* m = X.M.as<double2>();
+    // the eigensolvers read the lower triangle of the column-major view, i.e. the upper
+    // triangle of this row-major buffer: only those tiles are computed, the rest is zero
+    launch_zero(D.stream, X.G.get(), X.G.bytes());
     if (X.colside)  // G = M^H M
-        gemm_dense(D, 'C', 'N', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nr);
+        gemm_dense(D, 'C', 'N', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nr, true);
     else  // G = M M^H
-        gemm_dense(D, 'N', 'C', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nc);
+        gemm_dense(D, 'N', 'C', m, (int)X.nc, m, (int)X.nc, X.G.as<double2>(), n, n, n, (int)X.nc, true);
     // G is handed to the eigensolver at unit trace (a sector of a normalised state can
     // have any small norm); the eigenvalues are scaled back after the solve
     DevBuf sc(D.pool.get(), 1024 * sizeof(double)), nr2(D.pool.get(), sizeof(double));
@@ -192,40 +224,73 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
                   make_double2(1.0 / X.gscale, 0.0), make_double2(0.0, 0.0), (int64_t)n * n);
 }
 
-// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n, in one
-// rocSOLVER call (zheevd_strided_batched when there are several: its tridiagonalisation
-// is launch-latency bound, so batching sectors costs about as much as one).  The
-// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and
-// column j of the returned vectors is conj(v_j), i.e. row j of the buffer read row-major
-// is v_j^H (ascending eigenvalues).
-inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs) {
+// Side from which the split eigensolver replaces zheevd (below it zheevd is cheap).
+constexpr int kSplitEigMin = 32;
+// Reflectors per block of the back-transformation.
+constexpr int kReflectorBlock = 128;
+
+// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n.  The
+// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
+// eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
+// Split solver (n >= kSplitEigMin): one batched zhetrd for all sectors (its
+// tridiagonalisation is launch-latency bound, so batching sectors costs about as much as
+// one), then dstedc per sector, alternating between the two solver streams; the vectors
+// are formed later, only for the rows the truncation needs (gram_eigvecs).  zheevd (all
+// vectors at once, batched) below kSplitEigMin or when asked for.
+inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, bool heevd = false) {
     const int n = xs[0]->k;
     const int nb = (int)xs.size();
+    const size_t nn = (size_t)n * n;
+    const bool split = !heevd && n >= kSplitEigMin;
     DevBuf W(D.pool.get(), (size_t)n * nb * sizeof(double)), E(D.pool.get(), (size_t)n * nb * sizeof(double));
     DevBuf info(D.pool.get(), sizeof(rocblas_int) * nb);
-    DevBuf Gb;
+    DevBuf Gb, taub;
     rocblas_double_complex* gp;
     if (nb == 1) {
         gp = reinterpret_cast<rocblas_double_complex*>(xs[0]->G.get());
-        const rocblas_status st = rocsolver_zheevd(sol.h, rocblas_evect_original, rocblas_fill_upper,
-                                                   n, gp, n, W.as<double>(), E.as<double>(),
-                                                   info.as<rocblas_int>());
-        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
     } else {
-        const size_t nn = (size_t)n * n;
         Gb = DevBuf(D.pool.get(), nn * nb * sizeof(double2));
         for (int b = 0; b < nb; ++b)
             KV_HIP(hipMemcpyAsync(Gb.as<double2>() + b * nn, xs[b]->G.get(), nn * sizeof(double2),
                                   hipMemcpyDeviceToDevice, D.stream));
         gp = reinterpret_cast<rocblas_double_complex*>(Gb.get());
-        const rocblas_status st = rocsolver_zheevd_strided_batched(
-            sol.h, rocblas_evect_original, rocblas_fill_upper, n, gp, n, (rocblas_stride)nn,
-            W.as<double>(), n, E.as<double>(), n, info.as<rocblas_int>(), nb);
-        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd_strided_batched failed");
+    }
+    if (!split) {
+        const rocblas_status st =
+            nb == 1 ? rocsolver_zheevd(sol.h, rocblas_evect_original, rocblas_fill_lower, n, gp, n,
+                                       W.as<double>(), E.as<double>(), info.as<rocblas_int>())
+                    : rocsolver_zheevd_strided_batched(sol.h, rocblas_evect_original, rocblas_fill_lower,
+                                                       n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
+                                                       E.as<double>(), n, info.as<rocblas_int>(), nb);
+        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
+    } else {
+        taub = DevBuf(D.pool.get(), (size_t)n * nb * sizeof(double2));
+        auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
+        const rocblas_status st = rocsolver_zhetrd_strided_batched(
+            sol.h, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
+            E.as<double>(), n, tp, n, nb);
+        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
+        for (int b = 0; b < nb; ++b) xs[b]->Zt = DevBuf(D.pool.get(), nn * sizeof(double));
+        KV_HIP(hipEventRecord(sol.fork, D.stream));
+        KV_HIP(hipStreamWaitEvent(sol.s2, sol.fork, 0));
+        for (int b = 0; b < nb; ++b) {
+            const rocblas_status s2 = rocsolver_dstedc(
+                b % 2 ? sol.h2 : sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
+                E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
+            KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
+        }
+        KV_HIP(hipEventRecord(sol.join, sol.s2));
+        KV_HIP(hipStreamWaitEvent(D.stream, sol.join, 0));
+        for (int b = 0; b < nb; ++b) {
+            xs[b]->tau = DevBuf(D.pool.get(), (size_t)n * sizeof(double2));
+            KV_HIP(hipMemcpyAsync(xs[b]->tau.get(), taub.as<double2>() + (size_t)b * n,
+                                  (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
+        }
+    }
+    if (nb > 1)
         for (int b = 0; b < nb; ++b)
             KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
                                   hipMemcpyDeviceToDevice, D.stream));
-    }
     std::vector<double> w((size_t)n * nb);
     std::vector<rocblas_int> hinfo(nb);
     KV_HIP(hipMemcpyAsync(w.data(), W.get(), w.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
@@ -233,22 +298,145 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs) {
                           D.stream));
     D.sync();
     for (int b = 0; b < nb; ++b) {
-        KV_REQUIRE(hinfo[b] == 0, ERR_SOLVER, "zheevd did not converge");
-        xs[b]->s.resize(n);
+        KV_REQUIRE(hinfo[b] == 0, ERR_SOLVER, "Hermitian eigensolver did not converge");
+        SectorSvd& X = *xs[b];
+        X.s.resize(n);
         for (int j = 0; j < n; ++j)
-            xs[b]->s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0) * xs[b]->gscale);
+            X.s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0) * X.gscale);
+        if (split) {
+            X.Ev = DevBuf();
+            X.ev_lo = n;  // no vectors formed yet
+        } else {
+            X.Ev = std::move(X.G);  // all vectors, column j = conj(v_j)
+            X.ev_lo = 0;
+            X.tau = DevBuf();
+            X.Zt = DevBuf();
+        }
     }
 }
 
-// Form the nk leading singular vectors of a Gram-route sector.
-inline void gram_vectors(Device& D, SectorSvd& X, int nk) {
+// Rows [lo, k) of a Gram sector's ascending eigenvectors into Ev: the tridiagonal
+// eigenvectors Zt[:, lo:k] carried back through the reflectors, Z <- Q Z with
+// Q = H_0 H_1 ... H_{n-2}, applied block by block from the last one:
+// Z <- Z - V_b (T_b (V_b^H Z)) = Z - P_b Y_b,  Y_b = V_b^H Z,  P_b = V_b T_b.
+// Column-major throughout (the rocBLAS layout); V_b and P_b are zero above row b*bw + 1.
+// S_b = V_b^H V_b and Y_b have few outputs (bw x bw, bw x k) over up to n rows, so both run
+// split over row chunks on the grouped GEMM (one group per block and chunk, partial
+// products summed by a second kernel); P_b and the update Z -= P_b Y_b are rocBLAS GEMMs.
+inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
     const int n = X.k;
+    if (X.ev_lo <= lo) return;
+    KV_REQUIRE(X.Zt.get() != nullptr, ERR_INTERNAL, "Gram eigenvectors requested before the eigensolver");
+    const int k = n - lo;
+    DevBuf Z(D.pool.get(), (size_t)n * k * sizeof(double2));
+    launch_real_cols(D.stream, X.Zt.as<double>(), n, lo, k, n, Z.as<double2>());
+    const int nref = n - 1;
+    if (nref > 0 && k > 0) {
+        const int bw = kReflectorBlock, kc = kReflectorBlock;  // block width, row chunk
+        const int nblk = (nref + bw - 1) / bw;
+        const int nch_max = (nref + kc - 1) / kc;
+        const size_t vcols = (size_t)nblk * bw;
+        const int64_t sbw = (int64_t)bw * bw;
+        DevBuf V(D.pool.get(), (size_t)n * vcols * sizeof(double2));
+        DevBuf P(D.pool.get(), (size_t)n * vcols * sizeof(double2));
+        DevBuf S(D.pool.get(), (size_t)nblk * sbw * sizeof(double2));
+        DevBuf T(D.pool.get(), (size_t)nblk * sbw * sizeof(double2));
+        DevBuf Y(D.pool.get(), (size_t)bw * k * sizeof(double2));
+        DevBuf Sp(D.pool.get(), (size_t)nch_max * nblk * sbw * sizeof(double2));
+        DevBuf Yp(D.pool.get(), (size_t)nch_max * bw * k * sizeof(double2));
+        launch_hh_explicit(D.stream, X.G.as<double2>(), V.as<double2>(), n, (int)vcols);
+        // work lists: S chunks of every block first, then the Y chunks block by block.
+        // Row-major view: S_b^T = A B with A[c][i] = V[i, c] ('N', ld n) and B = conj(V) ('C');
+        // Y_b^T = Zr conj(V_b) with Zr[j][i] = Z[i, j] ('N', ld n).
+        std::vector<GemmGroup> gs;
+        std::vector<GemmSeg> ss;
+        std::vector<GemmTile> ts;
+        auto add = [&](int m, int64_t c_off, int64_t a_off, int64_t b_off, int kk) {
+            GemmGroup g{};
+            g.c_off = c_off;
+            g.m = m;
+            g.n = bw;
+            g.ldc = bw;
+            g.seg_begin = (int)ss.size();
+            g.seg_count = 1;
+            GemmSeg sg{};
+            sg.a_off = a_off;
+            sg.b_off = b_off;
+            sg.k = kk;
+            sg.lda = n;
+            sg.ldb = n;
+            ss.push_back(sg);
+            const int gi = (int)gs.size();
+            gs.push_back(g);
+            for (int a = 0; a < (m + GEMM_BM - 1) / GEMM_BM; ++a)
+                for (int b = 0; b < (bw + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{gi, a, b, 0});
+        };
+        std::vector<int> nch(nblk);
+        for (int b = 0; b < nblk; ++b) {
+            const int r0 = b * bw + 1, mb = n - r0;
+            nch[b] = (mb + kc - 1) / kc;
+            for (int c = 0; c < nch[b]; ++c)
+                add(bw, ((int64_t)c * nblk + b) * sbw, (int64_t)b * bw * n + r0 + (int64_t)c * kc,
+                    (int64_t)b * bw * n + r0 + (int64_t)c * kc, std::min(kc, mb - c * kc));
+        }
+        const int s_tiles = (int)ts.size();
+        std::vector<int> y_off(nblk), y_cnt(nblk);
+        for (int b = 0; b < nblk; ++b) {
+            const int r0 = b * bw + 1, mb = n - r0;
+            y_off[b] = (int)ts.size();
+            for (int c = 0; c < nch[b]; ++c)
+                add(k, (int64_t)c * bw * k, r0 + (int64_t)c * kc, (int64_t)b * bw * n + r0 + (int64_t)c * kc,
+                    std::min(kc, mb - c * kc));
+            y_cnt[b] = (int)ts.size() - y_off[b];
+        }
+        DevBuf dg = upload_vec(D, gs), dsg = upload_vec(D, ss), dt = upload_vec(D, ts);
+        const GemmTile* tiles = dt.as<GemmTile>();
+        launch_zero(D.stream, Sp.get(), Sp.bytes());  // blocks with fewer chunks leave parts unset
+        launch_gemm<double2>(D.stream, 0, 2, V.as<double2>(), V.as<double2>(), Sp.as<double2>(), tiles,
+                             s_tiles, dg.as<GemmGroup>(), dsg.as<GemmSeg>(), 1.0, 0.0);
+        launch_sum_parts(D.stream, Sp.as<double2>(), nch_max, (int64_t)nblk * sbw, S.as<double2>());
+        launch_hh_tfactor(D.stream, S.as<double2>(), X.tau.as<double2>(), T.as<double2>(), nblk, bw, nref);
+        auto* v = reinterpret_cast<rocblas_double_complex*>(V.get());
+        auto* p = reinterpret_cast<rocblas_double_complex*>(P.get());
+        auto* t = reinterpret_cast<rocblas_double_complex*>(T.get());
+        auto* y = reinterpret_cast<rocblas_double_complex*>(Y.get());
+        auto* z = reinterpret_cast<rocblas_double_complex*>(Z.get());
+        const rocblas_double_complex one(1.0, 0.0), zero(0.0, 0.0), mone(-1.0, 0.0);
+        const rocblas_stride sv = (rocblas_stride)n * bw;
+        auto ok = [](rocblas_status st) {
+            KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocBLAS zgemm failed");
+        };
+        // P_b = V_b T_b for every block at once
+        ok(rocblas_zgemm_strided_batched(sol.h, rocblas_operation_none, rocblas_operation_none, n, bw,
+                                         bw, &one, v, n, sv, t, bw, (rocblas_stride)sbw, &zero, p, n, sv,
+                                         nblk));
+        for (int b = nblk - 1; b >= 0; --b) {
+            const int r0 = b * bw + 1;
+            const int mb = n - r0;
+            if (mb <= 0) continue;
+            launch_gemm<double2>(D.stream, 0, 2, Z.as<double2>(), V.as<double2>(), Yp.as<double2>(),
+                                 tiles + y_off[b], y_cnt[b], dg.as<GemmGroup>(), dsg.as<GemmSeg>(), 1.0,
+                                 0.0);
+            launch_sum_parts(D.stream, Yp.as<double2>(), nch[b], (int64_t)bw * k, Y.as<double2>());
+            ok(rocblas_zgemm(sol.h, rocblas_operation_none, rocblas_operation_none, mb, k, bw, &mone,
+                             p + (size_t)b * sv + r0, n, y, bw, &one, z + r0, n));
+        }
+    }
+    D.sync();
+    X.Ev = std::move(Z);
+    X.ev_lo = lo;
+}
+
+// Form the nk leading singular vectors of a Gram-route sector.
+inline void gram_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
+    const int n = X.k;
+    gram_eigvecs(D, sol, X, n - nk);
     DevBuf B(D.pool.get(), (size_t)nk * n * sizeof(double2));  // rows v_j^H (or u_j^H), descending
     {
         std::vector<Copy2D> cps(nk);
         for (int j = 0; j < nk; ++j) {
             Copy2D c{};
-            c.src_off = (int64_t)(n - 1 - j) * n;
+            c.src_off = (int64_t)(n - 1 - j - X.ev_lo) * n;
             c.dst_off = (int64_t)j * n;
             c.rows = 1;
             c.cols = n;
@@ -257,7 +445,7 @@ inline void gram_vectors(Device& D, SectorSvd& X, int nk) {
             cps[j] = c;
         }
         DevBuf dc = upload_vec(D, cps);
-        launch_copy2d<double2>(D.stream, X.G.as<double2>(), B.as<double2>(), dc.as<Copy2D>(), nk, 1,
+        launch_copy2d<double2>(D.stream, X.Ev.as<double2>(), B.as<double2>(), dc.as<Copy2D>(), nk, 1,
                                nullptr);
     }
     std::vector<double> inv(nk);
@@ -301,7 +489,7 @@ inline void gram_vectors(Device& D, SectorSvd& X, int nk) {
 // tail * s_max (buffer rows [0, nt) in ascending order) span an accurate subspace; M
 // restricted to it (Mt = Bt M or M Bt^H) has norm ~ tail * s_max, and the eigenvectors Y
 // of its own Gram matrix rotate the tail rows: Bt <- Y^H Bt.
-inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
+inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, bool heevd) {
     X.refined = true;
     const int n = X.k;
     const int kmax = (int)std::min(X.nr, X.nc);
@@ -310,7 +498,8 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
     while (na < kmax && X.s[na] >= tail * smax) ++na;
     const int nt = n - na;
     if (nt <= 0 || na >= kmax) return;
-    double2* Bt = X.G.as<double2>();  // rows [0, nt) of the eigenvector buffer (ld n)
+    gram_eigvecs(D, sol, X, 0);
+    double2* Bt = X.Ev.as<double2>();  // rows [0, nt) of the eigenvector buffer (ld n)
     const double2* m = X.M.as<double2>();
     SectorSvd T;
     T.k = nt;
@@ -337,11 +526,12 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
         gemm_dense(D, 'C', 'N', Mt.as<double2>(), nt, Mt.as<double2>(), nt, T.G.as<double2>(),
                    nt, nt, nt, (int)X.nr);
     }
-    gram_solve(D, sol, {&T});
+    gram_solve(D, sol, {&T}, heevd);
+    gram_eigvecs(D, sol, T, 0);
     for (double& v : T.s) v *= (ref > 0 ? ref : 1.0);
     // Bt <- Ybuf Bt (rows y_j^H, ascending) and the tail values replaced
     DevBuf Bn(D.pool.get(), (size_t)nt * n * sizeof(double2));
-    gemm_dense(D, 'N', 'N', T.G.as<double2>(), nt, Bt, n, Bn.as<double2>(), n, nt, n, nt);
+    gemm_dense(D, 'N', 'N', T.Ev.as<double2>(), nt, Bt, n, Bn.as<double2>(), n, nt, n, nt);
     KV_HIP(hipMemcpyAsync(Bt, Bn.get(), (size_t)nt * n * sizeof(double2), hipMemcpyDeviceToDevice,
                           D.stream));
     D.sync();
@@ -350,14 +540,15 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail) {
 
 // Isometric route: the nk leading vectors are rows of the eigenvector buffer; the other
 // factor is one GEMM and carries S.
-inline void iso_vectors(Device& D, SectorSvd& X, int nk) {
+inline void iso_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
     const int n = X.k;
+    gram_eigvecs(D, sol, X, n - nk);
     DevBuf B(D.pool.get(), (size_t)nk * n * sizeof(double2));  // descending rows
     {
         std::vector<Copy2D> cps(nk);
         for (int j = 0; j < nk; ++j) {
             Copy2D c{};
-            c.src_off = (int64_t)(n - 1 - j) * n;
+            c.src_off = (int64_t)(n - 1 - j - X.ev_lo) * n;
             c.dst_off = (int64_t)j * n;
             c.rows = 1;
             c.cols = n;
@@ -366,7 +557,7 @@ inline void iso_vectors(Device& D, SectorSvd& X, int nk) {
             cps[j] = c;
         }
         DevBuf dc = upload_vec(D, cps);
-        launch_copy2d<double2>(D.stream, X.G.as<double2>(), B.as<double2>(), dc.as<Copy2D>(), nk, 1,
+        launch_copy2d<double2>(D.stream, X.Ev.as<double2>(), B.as<double2>(), dc.as<Copy2D>(), nk, 1,
                                nullptr);
     }
     X.s_absorbed = true;
@@ -447,7 +638,7 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         std::map<int, std::vector<SectorSvd*>> by_n;
         for (int p = 0; p < np; ++p)
             if (X[p].gram) by_n[X[p].k].push_back(&X[p]);
-        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second);
+        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_heevd);
         // the isometric route's values: only min(rows, cols) are singular values
         for (int p = 0; p < np; ++p)
             if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
@@ -470,14 +661,14 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             if (X[p].iso) {
                 if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
                     opt.tail * X[p].s[0] >= s_last) {
-                    iso_refine(D, sol, X[p], opt.tail);
+                    iso_refine(D, sol, X[p], opt.tail, opt.eig_heevd);
                     redo = true;
                 }
             } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
                 // U = M V / s would lose orthonormality at these values: redo the sector
                 // on its isometric (larger) side
                 gram_prepare(D, X[p], true, !left2right);
-                gram_solve(D, sol, {&X[p]});
+                gram_solve(D, sol, {&X[p]}, opt.eig_heevd);
                 X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
                 redo = true;
             }
@@ -496,10 +687,10 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         res.detail += buf;
         if (X[p].iso) {
             res.gram_sectors += 1;
-            if (kept[p] > 0) iso_vectors(D, X[p], kept[p]);
+            if (kept[p] > 0) iso_vectors(D, sol, X[p], kept[p]);
         } else if (X[p].gram) {
             res.gram_sectors += 1;
-            if (kept[p] > 0) gram_vectors(D, X[p], kept[p]);
+            if (kept[p] > 0) gram_vectors(D, sol, X[p], kept[p]);
         } else {
             res.exact_sectors += 1;
         }

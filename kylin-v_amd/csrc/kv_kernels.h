@@ -63,4 +63,17 @@ void launch_f64_to_f32(hipStream_t st, const double2* src, float2* dst, int64_t 
 // dst = s * src
 void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s, int64_t n);
 
+// Back-transformation of a lower-triangle Householder tridiagonalisation (LAPACK layout:
+// reflector i stored below the subdiagonal of column i, tau_i separately).
+// V (column-major n x ncols): the reflector vectors written out with their unit entries.
+void launch_hh_explicit(hipStream_t st, const double2* A, double2* V, int n, int ncols);
+// T (bw x bw per block, column-major, upper; bw <= 128): triangular factor of reflectors
+// [b bw, min((b+1) bw, nref)) from S_b = V_b^H V_b, so the block is I - V_b T_b V_b^H.
+void launch_hh_tfactor(hipStream_t st, const double2* S, const double2* tau, double2* T, int nblocks,
+                       int bw, int nref);
+// out[e] = sum_{p < np} part[p n + e]: partial products of a split-K GEMM summed.
+void launch_sum_parts(hipStream_t st, const double2* part, int np, int64_t n, double2* out);
+// Z[j n + i] = C[(c0 + j) ldc + i]  (i < n, j < k): real columns to complex.
+void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, int n, double2* Z);
+
 }  // namespace kv

@@ -904,4 +904,138 @@ void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s
     KV_HIP(hipGetLastError());
 }
 
+// ---- back-transformation of a Householder tridiagonalisation ------------------------
+// Lower-triangle reduction Q^H A Q = T of a column-major n x n matrix: reflector i < n-1 is
+// H_i = I - tau_i v_i v_i^H with v_i[r] = 0 for r <= i, 1 at r = i + 1 and A[i n + r] below.
+// V (column-major, n rows, ncols >= n - 1 columns) receives every v_i explicitly; columns
+// from n - 1 on are zero.
+__global__ __launch_bounds__(256) void hh_explicit_kernel(const double2* __restrict__ A,
+                                                          double2* __restrict__ V, int n,
+                                                          int ncols) {
+    const int64_t total = (int64_t)n * ncols;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int i = (int)(e / n), r = (int)(e % n);
+        double2 v = make_double2(0.0, 0.0);
+        if (i < n - 1) {
+            if (r == i + 1)
+                v.x = 1.0;
+            else if (r > i + 1)
+                v = A[e];
+        }
+        V[e] = v;
+    }
+}
+
+void launch_hh_explicit(hipStream_t st, const double2* A, double2* V, int n, int ncols) {
+    const int64_t total = (int64_t)n * ncols;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(hh_explicit_kernel, dim3(ew_blocks(total)), dim3(256), 0, st, A, V, n, ncols);
+    KV_HIP(hipGetLastError());
+}
+
+// Triangular factors of blocks of bw consecutive reflectors (forward, columnwise):
+// H_{i0} ... H_{i0+w-1} = I - V_b T_b V_b^H with T_b(c,c) = tau_c and
+// T_b(0:c, c) = -tau_c T_b(0:c, 0:c) S_b(0:c, c), S_b = V_b^H V_b.  S and T hold one bw x bw
+// column-major matrix per block; entries of columns c >= w (past the last reflector) are 0.
+// One workgroup per block; the upper triangle of T_b lives packed in LDS (column j at
+// j (j + 1) / 2).  Each new column is a triangular matrix-vector product: row r is summed
+// by kTfSplit threads over interleaved j, partial sums combined through LDS.
+constexpr int kTfactorMax = 128, kTfSplit = 4;
+__global__ __launch_bounds__(kTfactorMax * kTfSplit) void hh_tfactor_kernel(
+    const double2* __restrict__ S, const double2* __restrict__ tau, double2* __restrict__ T, int bw,
+    int nref) {
+    __shared__ double2 tp[kTfactorMax * (kTfactorMax + 1) / 2];
+    __shared__ double2 sc[kTfactorMax];
+    __shared__ double2 part[kTfSplit][kTfactorMax];
+    const int b = blockIdx.x;
+    const int i0 = b * bw;
+    const int w = min(bw, nref - i0);
+    const int r = threadIdx.x % kTfactorMax, q = threadIdx.x / kTfactorMax;
+    const double2* Sb = S + (int64_t)b * bw * bw;
+    double2* Tb = T + (int64_t)b * bw * bw;
+    for (int c = 0; c < w; ++c) {
+        if (q == 0 && r < c) sc[r] = Sb[(int64_t)c * bw + r];
+        __syncthreads();
+        // partial of sum_{j=r}^{c-1} T(r, j) S(j, c) over j = r + q, r + q + kTfSplit, ...
+        double sr = 0.0, si = 0.0;
+        if (r < c) {
+#pragma unroll 4
+            for (int j = r + q; j < c; j += kTfSplit) {
+                const double2 a = tp[j * (j + 1) / 2 + r], x = sc[j];
+                sr += a.x * x.x - a.y * x.y;
+                si += a.x * x.y + a.y * x.x;
+            }
+        }
+        part[q][r] = make_double2(sr, si);
+        __syncthreads();
+        if (q == 0) {
+            const double2 t = tau[i0 + c];
+            if (r < c) {
+                double ar = 0.0, ai = 0.0;
+#pragma unroll
+                for (int u = 0; u < kTfSplit; ++u) {
+                    ar += part[u][r].x;
+                    ai += part[u][r].y;
+                }
+                tp[c * (c + 1) / 2 + r] = make_double2(-(t.x * ar - t.y * ai), -(t.x * ai + t.y * ar));
+            } else if (r == c) {
+                tp[c * (c + 1) / 2 + c] = t;
+            }
+        }
+        __syncthreads();  // column c complete before it is read; sc and part free again
+    }
+    for (int e = threadIdx.x; e < bw * bw; e += blockDim.x) {
+        const int c = e / bw, rr = e % bw;
+        Tb[e] = (c < w && rr <= c) ? tp[c * (c + 1) / 2 + rr] : make_double2(0.0, 0.0);
+    }
+}
+
+void launch_hh_tfactor(hipStream_t st, const double2* S, const double2* tau, double2* T, int nblocks,
+                       int bw, int nref) {
+    if (nblocks <= 0) return;
+    KV_REQUIRE(bw >= 1 && bw <= kTfactorMax, ERR_ARG, "reflector block width must be in [1, 128]");
+    KV_REQUIRE((int64_t)nblocks * bw >= nref && (int64_t)(nblocks - 1) * bw < nref, ERR_ARG,
+               "reflector blocks do not cover the reflectors");
+    hipLaunchKernelGGL(hh_tfactor_kernel, dim3(nblocks), dim3(kTfactorMax * kTfSplit), 0, st, S, tau, T, bw,
+                       nref);
+    KV_HIP(hipGetLastError());
+}
+
+// out[e] = sum_{p < np} part[p * n + e]  (split-K partial products summed).
+__global__ __launch_bounds__(256) void sum_parts_kernel(const double2* __restrict__ part, int np,
+                                                        int64_t n, double2* __restrict__ out) {
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        double re = 0, im = 0;
+        for (int p = 0; p < np; ++p) {
+            const double2 v = part[(int64_t)p * n + e];
+            re += v.x;
+            im += v.y;
+        }
+        out[e] = make_double2(re, im);
+    }
+}
+
+void launch_sum_parts(hipStream_t st, const double2* part, int np, int64_t n, double2* out) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(sum_parts_kernel, dim3(ew_blocks(n)), dim3(256), 0, st, part, np, n, out);
+    KV_HIP(hipGetLastError());
+}
+
+// Z[j n + i] = C[(c0 + j) ldc + i] for i < n, j < k (real columns to complex).
+__global__ __launch_bounds__(256) void real_cols_kernel(const double* __restrict__ C, int ldc, int c0,
+                                                        int k, int n, double2* __restrict__ Z) {
+    const int64_t total = (int64_t)n * k;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int j = (int)(e / n), i = (int)(e % n);
+        Z[e] = make_double2(C[(int64_t)(c0 + j) * ldc + i], 0.0);
+    }
+}
+
+void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, int n, double2* Z) {
+    const int64_t total = (int64_t)n * k;
+    if (total <= 0) return;
+    hipLaunchKernelGGL(real_cols_kernel, dim3(ew_blocks(total)), dim3(256), 0, st, C, ldc, c0, k, n, Z);
+    KV_HIP(hipGetLastError());
+}
+
 }  // namespace kv
