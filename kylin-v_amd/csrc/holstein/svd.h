@@ -52,32 +52,17 @@ inline std::mutex& solver_init_mutex() {
     return m;
 }
 
-// h runs on the context stream; h2 on a second stream of default priority, used to overlap
-// the per-matrix tridiagonal eigensolvers of a batch (fork/join by events).
 struct Solver {
-    rocblas_handle h = nullptr, h2 = nullptr;
-    hipStream_t s2 = nullptr;
-    hipEvent_t fork = nullptr, join = nullptr;
+    rocblas_handle h = nullptr;
     explicit Solver(Device& dev) {
         std::lock_guard<std::mutex> g(solver_init_mutex());
         if (rocblas_create_handle(&h) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_create_handle failed");
         if (rocblas_set_stream(h, dev.stream) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_set_stream failed");
-        KV_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
-        KV_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
-        KV_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
-        if (rocblas_create_handle(&h2) != rocblas_status_success)
-            throw Error(ERR_SOLVER, "rocblas_create_handle failed");
-        if (rocblas_set_stream(h2, s2) != rocblas_status_success)
-            throw Error(ERR_SOLVER, "rocblas_set_stream failed");
     }
     ~Solver() {
         if (h) rocblas_destroy_handle(h);
-        if (h2) rocblas_destroy_handle(h2);
-        if (s2) (void)hipStreamDestroy(s2);
-        if (fork) (void)hipEventDestroy(fork);
-        if (join) (void)hipEventDestroy(join);
     }
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
@@ -92,16 +77,24 @@ struct SvdResult {
     std::string detail;  // per sector: rows x cols, route, s_max, last kept value
 };
 
+// Side from which the split eigensolver replaces zheevd by default.  Below it the
+// back-transformation of all vectors inside zheevd is cheap, and the split route's extra
+// launches and host synchronisations cost more than they save: with the split route at
+// every side, 8 concurrent chains of the batching workload ran 7% slower.
+constexpr int kSplitEigMin = 1024;
+enum EigMode { EIG_AUTO = 0, EIG_SPLIT = 1, EIG_HEEVD = 2 };
+
 struct SvdOptions {
     double guard = 1e-4;    // smaller-side Gram vectors only if s_kept_min >= guard * s_max
     int iso_min = 64;       // smaller side from which the Gram routes are used
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
-    bool eig_heevd = false;  // Gram eigenproblems by zheevd instead of the split solver
+    int eig_mode = EIG_AUTO;  // Gram eigensolver: split from kSplitEigMin, or forced
     static SvdOptions from_env() {
         SvdOptions o;
         if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
-        if (const char* e = std::getenv("KV_EIG")) o.eig_heevd = std::string(e) == "heevd";
+        if (const char* e = std::getenv("KV_EIG"))
+            o.eig_mode = std::string(e) == "heevd" ? EIG_HEEVD : std::string(e) == "split" ? EIG_SPLIT : EIG_AUTO;
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
         if (const char* m = std::getenv("KV_ISO_MIN")) o.iso_min = std::atoi(m);
         if (const char* t = std::getenv("KV_ISO_TAIL")) o.tail = std::atof(t);
@@ -224,24 +217,23 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
                   make_double2(1.0 / X.gscale, 0.0), make_double2(0.0, 0.0), (int64_t)n * n);
 }
 
-// Side from which the split eigensolver replaces zheevd (below it zheevd is cheap).
-constexpr int kSplitEigMin = 32;
 // Reflectors per block of the back-transformation.
 constexpr int kReflectorBlock = 128;
 
 // Eigen-decomposition of the Gram matrices of one or more sectors of equal size n.  The
 // row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
 // eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
-// Split solver (n >= kSplitEigMin): one batched zhetrd for all sectors (its
+// Split solver (by default n >= kSplitEigMin): one batched zhetrd for all sectors (its
 // tridiagonalisation is launch-latency bound, so batching sectors costs about as much as
-// one), then dstedc per sector, alternating between the two solver streams; the vectors
-// are formed later, only for the rows the truncation needs (gram_eigvecs).  zheevd (all
-// vectors at once, batched) below kSplitEigMin or when asked for.
-inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, bool heevd = false) {
+// one), then dstedc per sector; the vectors are formed later, only for the rows the
+// truncation needs (gram_eigvecs).  zheevd (all vectors at once, batched) otherwise.  (dstedc of a pair on two streams measured slower in
+// the sweep than one after the other: the cross-stream event waits cost more than the
+// overlap.)
+inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO) {
     const int n = xs[0]->k;
     const int nb = (int)xs.size();
     const size_t nn = (size_t)n * n;
-    const bool split = !heevd && n >= kSplitEigMin;
+    const bool split = mode == EIG_SPLIT ? n >= 2 : mode == EIG_AUTO && n >= kSplitEigMin;
     DevBuf W(D.pool.get(), (size_t)n * nb * sizeof(double)), E(D.pool.get(), (size_t)n * nb * sizeof(double));
     DevBuf info(D.pool.get(), sizeof(rocblas_int) * nb);
     DevBuf Gb, taub;
@@ -270,17 +262,13 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, bool 
             sol.h, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
             E.as<double>(), n, tp, n, nb);
         KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
-        for (int b = 0; b < nb; ++b) xs[b]->Zt = DevBuf(D.pool.get(), nn * sizeof(double));
-        KV_HIP(hipEventRecord(sol.fork, D.stream));
-        KV_HIP(hipStreamWaitEvent(sol.s2, sol.fork, 0));
         for (int b = 0; b < nb; ++b) {
+            xs[b]->Zt = DevBuf(D.pool.get(), nn * sizeof(double));
             const rocblas_status s2 = rocsolver_dstedc(
-                b % 2 ? sol.h2 : sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
+                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
                 E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
-        KV_HIP(hipEventRecord(sol.join, sol.s2));
-        KV_HIP(hipStreamWaitEvent(D.stream, sol.join, 0));
         for (int b = 0; b < nb; ++b) {
             xs[b]->tau = DevBuf(D.pool.get(), (size_t)n * sizeof(double2));
             KV_HIP(hipMemcpyAsync(xs[b]->tau.get(), taub.as<double2>() + (size_t)b * n,
@@ -489,7 +477,7 @@ inline void gram_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
 // tail * s_max (buffer rows [0, nt) in ascending order) span an accurate subspace; M
 // restricted to it (Mt = Bt M or M Bt^H) has norm ~ tail * s_max, and the eigenvectors Y
 // of its own Gram matrix rotate the tail rows: Bt <- Y^H Bt.
-inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, bool heevd) {
+inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, int mode) {
     X.refined = true;
     const int n = X.k;
     const int kmax = (int)std::min(X.nr, X.nc);
@@ -526,7 +514,7 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, bool h
         gemm_dense(D, 'C', 'N', Mt.as<double2>(), nt, Mt.as<double2>(), nt, T.G.as<double2>(),
                    nt, nt, nt, (int)X.nr);
     }
-    gram_solve(D, sol, {&T}, heevd);
+    gram_solve(D, sol, {&T}, mode);
     gram_eigvecs(D, sol, T, 0);
     for (double& v : T.s) v *= (ref > 0 ? ref : 1.0);
     // Bt <- Ybuf Bt (rows y_j^H, ascending) and the tail values replaced
@@ -638,7 +626,7 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         std::map<int, std::vector<SectorSvd*>> by_n;
         for (int p = 0; p < np; ++p)
             if (X[p].gram) by_n[X[p].k].push_back(&X[p]);
-        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_heevd);
+        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_mode);
         // the isometric route's values: only min(rows, cols) are singular values
         for (int p = 0; p < np; ++p)
             if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
@@ -661,14 +649,14 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             if (X[p].iso) {
                 if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
                     opt.tail * X[p].s[0] >= s_last) {
-                    iso_refine(D, sol, X[p], opt.tail, opt.eig_heevd);
+                    iso_refine(D, sol, X[p], opt.tail, opt.eig_mode);
                     redo = true;
                 }
             } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
                 // U = M V / s would lose orthonormality at these values: redo the sector
                 // on its isometric (larger) side
                 gram_prepare(D, X[p], true, !left2right);
-                gram_solve(D, sol, {&X[p]}, opt.eig_heevd);
+                gram_solve(D, sol, {&X[p]}, opt.eig_mode);
                 X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
                 redo = true;
             }
