@@ -4,7 +4,8 @@
 //    launch covers every output block of a contraction stage; a group sums its K
 //    segments (the block pairs that land on the same output block) inside one tile,
 //    so there are no atomics and results are deterministic.  Complex products run as
-//    four real MFMAs (re.re - im.im, re.im + im.re) on v_mfma_f64_16x16x4f64 or
+//    four real MFMAs (re.re - im.im, re.im + im.re) on v_mfma_f64_16x16x4f64, and in
+//    FP32 as three (re.re, im.im, (re+im)(re+im): Gauss's product) on
 //    v_mfma_f32_16x16x4f32; 64x64 complex tile per 256-thread workgroup, 4 waves in
 //    a 2x2 arrangement of 32x32 complex wave tiles, operands staged through LDS.
 //  * wfold: applies the sparse MPO tensor as a gather-sum over contiguous slices.
@@ -72,13 +73,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
     const int wm = (wave >> 1) * (BM / 2), wn = (wave & 1) * (BN / 2);
     const int lr = lane & 15, lk = lane >> 4;
 
-    acc_t cre[SI][SJ], cim[SI][SJ];
+    // FP64: cre = sum re.re - im.im, cim = sum re.im + im.re (four MFMAs per step).
+    // FP32: three MFMAs per step (Gauss): cre = sum re.re, cp2 = sum im.im,
+    // cim = sum (re + im)(re + im); the epilogue forms re = cre - cp2, im = cim - cre - cp2.
+    constexpr bool kGauss = sizeof(T) == sizeof(float);
+    acc_t cre[SI][SJ], cim[SI][SJ], cp2[SI][SJ];
 #pragma unroll
     for (int i = 0; i < SI; ++i)
 #pragma unroll
         for (int j = 0; j < SJ; ++j) {
             cre[i][j] = acc_t{0, 0, 0, 0};
             cim[i][j] = acc_t{0, 0, 0, 0};
+            cp2[i][j] = acc_t{0, 0, 0, 0};
         }
 
     // K stages run over (segment, k0) pairs.  The next stage's operands are loaded into
@@ -174,15 +180,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
             for (int i = 0; i < SI; ++i) av[i] = As[buf][kq + lk][wm + i * 16 + lr];
 #pragma unroll
             for (int j = 0; j < SJ; ++j) bv[j] = Bs[buf][kq + lk][wn + j * 16 + lr];
+            if constexpr (kGauss) {
+                T as[SI], bs[SJ];
 #pragma unroll
-            for (int i = 0; i < SI; ++i)
+                for (int i = 0; i < SI; ++i) as[i] = av[i].x + av[i].y;
 #pragma unroll
-                for (int j = 0; j < SJ; ++j) {
-                    cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
-                    cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
-                    cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
-                    cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
-                }
+                for (int j = 0; j < SJ; ++j) bs[j] = bv[j].x + bv[j].y;
+#pragma unroll
+                for (int i = 0; i < SI; ++i)
+#pragma unroll
+                    for (int j = 0; j < SJ; ++j) {
+                        cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
+                        cp2[i][j] = Cx<T>::mfma(av[i].y, bv[j].y, cp2[i][j]);
+                        cim[i][j] = Cx<T>::mfma(as[i], bs[j], cim[i][j]);
+                    }
+            } else {
+#pragma unroll
+                for (int i = 0; i < SI; ++i)
+#pragma unroll
+                    for (int j = 0; j < SJ; ++j) {
+                        cre[i][j] = Cx<T>::mfma(av[i].x, bv[j].x, cre[i][j]);
+                        cre[i][j] = Cx<T>::mfma(-av[i].y, bv[j].y, cre[i][j]);
+                        cim[i][j] = Cx<T>::mfma(av[i].x, bv[j].y, cim[i][j]);
+                        cim[i][j] = Cx<T>::mfma(av[i].y, bv[j].x, cim[i][j]);
+                    }
+            }
         }
         if (s2 < g.seg_count) store(buf ^ 1);
         __syncthreads();
@@ -200,7 +222,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
                 const int row = m0 + wm + i * 16 + Cx<T>::out_row(lk, v);
                 const int col = n0 + wn + j * 16 + lr;
                 if (row < g.m && col < g.n) {
-                    const T re = cre[i][j][v], im = cim[i][j][v];
+                    T re = cre[i][j][v], im = cim[i][j][v];
+                    if constexpr (kGauss) {
+                        const T p1 = re, p2 = cp2[i][j][v];
+                        re = p1 - p2;
+                        im = im - p1 - p2;
+                    }
                     T2 o;
                     o.x = alpha_re * re - alpha_im * im;
                     o.y = alpha_re * im + alpha_im * re;
