@@ -124,6 +124,21 @@ int kv_heff_apply(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W
                   const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x,
                   int precision, kv_tensor** y);
 
+/* H_eff with its sector-pairing lists built once per bond and kept on the device: the
+ * build half of apply_double / apply_single / apply_zero (holstein/product.h:478-553),
+ * which the reference repeats on every Lanczos iteration (holstein/exponential.h:113-124).
+ * x gives the block structure; the plan's vectors have the union structure of x and
+ * H_eff(x) (the output structure of kv_heff_apply).  L and R must outlive the plan (the
+ * FP64 contractions read them at every apply); W1 and W2 are folded into the plan. */
+typedef struct kv_heff_plan kv_heff_plan;
+int kv_heff_plan_create(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                        const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x,
+                        int precision, kv_heff_plan** out);
+/* y = scale * H_eff x, a new tensor with the plan's structure.  x with that structure (a
+ * previous output) is read in place; x with a sub-structure is first spread into it. */
+int kv_heff_plan_apply(kv_heff_plan* plan, const kv_tensor* x, double scale, kv_tensor** y);
+int kv_heff_plan_destroy(kv_heff_plan* plan);
+
 /* env_sweep(env, A, W, Left2Right) (holstein/product.h:488-503); with bra != NULL the
  * bra/ket form env_sweep(env, bra, W, ket) (holstein/product.h:505-521). */
 int kv_env_update(kv_ctx* ctx, const kv_tensor* env, const kv_tensor* bra,

@@ -80,6 +80,9 @@ SIGNATURES = {
     "kv_tensor_axpy": (_I, [_P, _P, _P, _D, _D, _PP]),
     "kv_tensor_scale": (_I, [_P, _P, _D, _D]),
     "kv_heff_apply": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _PP]),
+    "kv_heff_plan_create": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _PP]),
+    "kv_heff_plan_apply": (_I, [_P, _P, _D, _PP]),
+    "kv_heff_plan_destroy": (_I, [_P]),
     "kv_env_update": (_I, [_P, _P, _P, _P, _P, _I, _I, _PP]),
     "kv_krylov_step": (_I, [_P, _I, _P, _P, _P, _P, _P, _D, _I, _D, _I, _I,
                             ctypes.POINTER(_KrylovExit)]),
@@ -305,6 +308,38 @@ def _heff(nsite, L, W1, W2, R, x, precision):
                                     W2.h if W2 is not None else None, R.h, x.h,
                                     _PREC[precision], ctypes.byref(h)))
     return Tensor(ctx, h)
+
+
+class HeffPlan:
+    """H_eff built once per bond (kv_heff_plan_create): apply() is y = scale * H_eff x with
+    the pairing lists kept on the device, as a Lanczos loop calls it.  The environments
+    must stay alive as long as the plan."""
+
+    def __init__(self, nsite, envL, x, mpo1, mpo2, envR, precision="fp64"):
+        ctx = x.ctx
+        self.ctx = ctx
+        self._keep = (envL, envR)
+        h = ctypes.c_void_p()
+        ctx.check(ctx.lib.kv_heff_plan_create(ctx.h, nsite, envL.h, mpo1.h if mpo1 is not None else None,
+                                              mpo2.h if mpo2 is not None else None, envR.h, x.h,
+                                              _PREC[precision], ctypes.byref(h)))
+        self.h = h
+
+    def apply(self, x, scale=1.0):
+        y = ctypes.c_void_p()
+        self.ctx.check(self.ctx.lib.kv_heff_plan_apply(self.h, x.h, float(scale), ctypes.byref(y)))
+        return Tensor(self.ctx, y)
+
+    def close(self):
+        if getattr(self, "h", None) and self.ctx.h:
+            self.ctx.lib.kv_heff_plan_destroy(self.h)
+        self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def heff_profile(nsite, envL, x, mpo1, mpo2, envR, precision="fp64", reps=5):

@@ -29,6 +29,11 @@ struct kv_tdvp {
     std::unique_ptr<kv::TDVP> s;
 };
 
+struct kv_heff_plan {
+    kv_ctx* ctx;
+    kv::Heff H;
+};
+
 namespace {
 
 template <class F>
@@ -271,6 +276,53 @@ int kv_heff_apply(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W
         ctx->dev->sync();
         *y = wrap(ctx, std::move(out));
     });
+}
+
+int kv_heff_plan_create(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,
+                        const kv_tensor* W2, const kv_tensor* R, const kv_tensor* x, int precision,
+                        kv_heff_plan** out) {
+    if (!ctx || !out) return KV_ERR_ARG;
+    *out = nullptr;
+    return guard(ctx, [&] {
+        KV_REQUIRE(nsite >= 0 && nsite <= 2, kv::ERR_ARG, "nsite must be 0, 1 or 2");
+        kv::Tensor* w1 = nsite >= 1 ? need(W1, "W1") : nullptr;
+        kv::Tensor* w2 = nsite == 2 ? need(W2, "W2") : nullptr;
+        std::unique_ptr<kv_heff_plan> p(new kv_heff_plan);
+        p->ctx = ctx;
+        p->H.build(ctx->dev, nsite, *need(L, "L"), w1, w2, *need(R, "R"), need(x, "x")->lay,
+                   prec_of(precision));
+        ctx->dev->sync();
+        *out = p.release();
+    });
+}
+
+int kv_heff_plan_apply(kv_heff_plan* plan, const kv_tensor* x, double scale, kv_tensor** y) {
+    if (!plan || !y) return KV_ERR_ARG;
+    kv_ctx* ctx = plan->ctx;
+    return guard(ctx, [&] {
+        const kv::Tensor* xt = need(x, "x");
+        KV_REQUIRE(x->ctx == ctx, kv::ERR_ARG, "x belongs to another context");
+        std::unique_ptr<kv::Tensor> spread;
+        if (!kv::same_layout(xt->lay, plan->H.layout())) {
+            spread = kv::relayout(ctx->dev, *xt, plan->H.layout());
+            xt = spread.get();
+        }
+        auto out = kv::Tensor::create(ctx->dev, plan->H.layout(), false);
+        plan->H.apply(xt->d(), out->d(), scale);
+        ctx->dev->sync();
+        *y = wrap(ctx, std::move(out));
+    });
+}
+
+int kv_heff_plan_destroy(kv_heff_plan* plan) {
+    if (!plan) return KV_ERR_ARG;
+    try {
+        plan->ctx->dev->sync();
+        delete plan;
+        return KV_OK;
+    } catch (...) {
+        return KV_ERR_INTERNAL;
+    }
 }
 
 int kv_heff_profile(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* W1,

@@ -165,6 +165,54 @@ def test_heff_large_ragged_sectors(kv, precision, monkeypatch):
     assert max_diff(got2, y2) <= tol * rounding_scale(T.apply_double, L, x2, W, W, R)
 
 
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_heff_plan_reused_like_lanczos(kv, precision):
+    """An H_eff plan built once (kv_heff_plan_create) and applied repeatedly: the first apply
+    equals the one-shot kv_heff_apply, scale is applied, and feeding each output back in
+    (as a Lanczos loop does) tracks the oracle's repeated apply_double / apply_single /
+    apply_zero."""
+    print(f"case heff plan {precision}", flush=True)
+    rng = np.random.default_rng(8)
+    a_l = {0: 70, 1: 33}
+    s_l = {0: 5, 1: 3}
+    w_l = {-1: 1, 0: 2, 1: 1}
+    b_l = {0: 45, 1: 97}
+    L = rand_block_tensor(rng, (1, -1, -1), [a_l, w_l, a_l])
+    W = rand_block_tensor(rng, (1, 1, -1, -1), [s_l, w_l, s_l, w_l])
+    R = rand_block_tensor(rng, (-1, 1, 1), [b_l, w_l, b_l])
+    R0 = rand_block_tensor(rng, (-1, 1, 1), [a_l, w_l, a_l])
+    cases = [
+        (2, rand_block_tensor(rng, (1, 1, 1, -1), [a_l, s_l, s_l, b_l]), W, W, R,
+         lambda x: T.apply_double(L, x, W, W, R)),
+        (1, rand_block_tensor(rng, (1, 1, -1), [a_l, s_l, b_l]), W, None, R,
+         lambda x: T.apply_single(L, x, W, R)),
+        (0, rand_block_tensor(rng, (1, -1), [a_l, a_l]), None, None, R0,
+         lambda x: T.apply_zero(L, x, R0)),
+    ]
+    tol = {"fp64": 1e-12, "fp32": 1e-4}[precision]  # of max|y| on these random tensors
+    gL = up(kv, L)
+    for nsite, x, w1, w2, r, ref_apply in cases:
+        gx, gr = up(kv, x), up(kv, r)
+        gw1 = up(kv, w1) if w1 is not None else None
+        gw2 = up(kv, w2) if w2 is not None else None
+        plan = kv.HeffPlan(nsite, gL, gx, gw1, gw2, gr, precision)
+        one_shot = down(kv._heff(nsite, gL, gw1, gw2, gr, gx, precision))
+        y1 = plan.apply(gx, 1.0)
+        assert max_diff(down(y1), one_shot) <= 1e-14 * scale(one_shot)
+        y = plan.apply(gx, 0.5)
+        r1 = ref_apply(x)
+        ref = OT(r1.dirs, {q: 0.5 * b for q, b in r1.blocks.items()})
+        assert max_diff(down(y), ref) <= tol * scale(ref)
+        # two more applies on the plan's own outputs
+        yg = y
+        for _ in range(2):
+            r1 = ref_apply(ref)
+            ref = OT(r1.dirs, {q: 0.5 * b for q, b in r1.blocks.items()})
+            yg = plan.apply(yg, 0.5)
+            assert max_diff(down(yg), ref) <= 10 * tol * scale(ref)
+        plan.close()
+
+
 @pytest.mark.parametrize("layout", ["merged", "pp"])
 def test_env_update_matches_oracle(kv, layout):
     print(f"case env {layout}", flush=True)
