@@ -229,6 +229,19 @@ constexpr int kReflectorBlock = 128;
 // truncation needs (gram_eigvecs).  zheevd (all vectors at once, batched) otherwise.  (dstedc of a pair on two streams measured slower in
 // the sweep than one after the other: the cross-stream event waits cost more than the
 // overlap.)
+// Sum of squares of `count` doubles on the device (NaN or Inf if any is).
+inline double dev_sumsq(Device& D, const double* p, int64_t count) {
+    DevBuf sc(D.pool.get(), 2048 * sizeof(double)), out(D.pool.get(), sizeof(double));
+    double tail = 0.0, v = 0.0;
+    if (count >= 2) launch_nrm2sq(D.stream, reinterpret_cast<const double2*>(p), count / 2, out.as<double>(),
+                                  sc.as<double>());
+    if (count % 2)
+        KV_HIP(hipMemcpyAsync(&tail, p + count - 1, sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    if (count >= 2) KV_HIP(hipMemcpyAsync(&v, out.get(), sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    D.sync();
+    return v + tail * tail;
+}
+
 inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO) {
     const int n = xs[0]->k;
     const int nb = (int)xs.size();
@@ -236,25 +249,48 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
     const bool split = mode == EIG_SPLIT ? n >= 2 : mode == EIG_AUTO && n >= kSplitEigMin;
     DevBuf W(D.pool.get(), (size_t)n * nb * sizeof(double)), E(D.pool.get(), (size_t)n * nb * sizeof(double));
     DevBuf info(D.pool.get(), sizeof(rocblas_int) * nb);
-    DevBuf Gb, taub;
-    rocblas_double_complex* gp;
-    if (nb == 1) {
-        gp = reinterpret_cast<rocblas_double_complex*>(xs[0]->G.get());
-    } else {
-        Gb = DevBuf(D.pool.get(), nn * nb * sizeof(double2));
-        for (int b = 0; b < nb; ++b)
-            KV_HIP(hipMemcpyAsync(Gb.as<double2>() + b * nn, xs[b]->G.get(), nn * sizeof(double2),
-                                  hipMemcpyDeviceToDevice, D.stream));
-        gp = reinterpret_cast<rocblas_double_complex*>(Gb.get());
-    }
+    // the solvers overwrite their input: they work on copies, so that a sector the divide
+    // and conquer fails on can be redone from its Gram matrix (or tridiagonal form)
+    DevBuf Gb(D.pool.get(), nn * nb * sizeof(double2)), taub;
+    for (int b = 0; b < nb; ++b)
+        KV_HIP(hipMemcpyAsync(Gb.as<double2>() + b * nn, xs[b]->G.get(), nn * sizeof(double2),
+                              hipMemcpyDeviceToDevice, D.stream));
+    auto* gp = reinterpret_cast<rocblas_double_complex*>(Gb.get());
+    std::vector<rocblas_int> hinfo(nb);
+    auto read_info = [&] {
+        KV_HIP(hipMemcpyAsync(hinfo.data(), info.get(), nb * sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                              D.stream));
+        D.sync();
+    };
+    // rocSOLVER's divide and conquer (inside zheevd, and dstedc) can fail on a tridiagonal
+    // that splits into exactly decoupled blocks (rank-deficient Gram matrices, e.g. of a
+    // product state): it reports non-convergence, or returns NaN eigenvectors with info 0.
+    // Such sectors are redone by implicit QR iteration (zheev / dsteqr).
     if (!split) {
-        const rocblas_status st =
-            nb == 1 ? rocsolver_zheevd(sol.h, rocblas_evect_original, rocblas_fill_lower, n, gp, n,
-                                       W.as<double>(), E.as<double>(), info.as<rocblas_int>())
-                    : rocsolver_zheevd_strided_batched(sol.h, rocblas_evect_original, rocblas_fill_lower,
-                                                       n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
-                                                       E.as<double>(), n, info.as<rocblas_int>(), nb);
+        const rocblas_status st = rocsolver_zheevd_strided_batched(
+            sol.h, rocblas_evect_original, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
+            E.as<double>(), n, info.as<rocblas_int>(), nb);
         KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
+        read_info();
+        for (int b = 0; b < nb; ++b) {
+            double2* slot = Gb.as<double2>() + b * nn;
+            const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, reinterpret_cast<const double*>(slot),
+                                                                      2 * (int64_t)nn));
+            if (ok) continue;
+            KV_HIP(hipMemcpyAsync(slot, xs[b]->G.get(), nn * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
+            const rocblas_status s2 = rocsolver_zheev(sol.h, rocblas_evect_original, rocblas_fill_lower, n,
+                                                      reinterpret_cast<rocblas_double_complex*>(slot), n,
+                                                      W.as<double>() + (size_t)b * n, E.as<double>() + (size_t)b * n,
+                                                      info.as<rocblas_int>() + b);
+            KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_zheev failed");
+            read_info();
+            KV_REQUIRE(hinfo[b] == 0 && std::isfinite(dev_sumsq(D, reinterpret_cast<const double*>(slot),
+                                                                 2 * (int64_t)nn)),
+                       ERR_SOLVER, "Hermitian eigensolver did not converge");
+        }
+        for (int b = 0; b < nb; ++b)
+            KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
+                                  hipMemcpyDeviceToDevice, D.stream));
     } else {
         taub = DevBuf(D.pool.get(), (size_t)n * nb * sizeof(double2));
         auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
@@ -262,6 +298,10 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             sol.h, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
             E.as<double>(), n, tp, n, nb);
         KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
+        // the tridiagonal forms, kept for a redo (dstedc overwrites both arrays)
+        DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
+        KV_HIP(hipMemcpyAsync(W0.get(), W.get(), W.bytes(), hipMemcpyDeviceToDevice, D.stream));
+        KV_HIP(hipMemcpyAsync(E0.get(), E.get(), E.bytes(), hipMemcpyDeviceToDevice, D.stream));
         for (int b = 0; b < nb; ++b) {
             xs[b]->Zt = DevBuf(D.pool.get(), nn * sizeof(double));
             const rocblas_status s2 = rocsolver_dstedc(
@@ -269,28 +309,42 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                 E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
+        read_info();
+        for (int b = 0; b < nb; ++b) {
+            const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), (int64_t)nn));
+            if (ok) continue;
+            KV_HIP(hipMemcpyAsync(W.as<double>() + (size_t)b * n, W0.as<double>() + (size_t)b * n, n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, D.stream));
+            KV_HIP(hipMemcpyAsync(E.as<double>() + (size_t)b * n, E0.as<double>() + (size_t)b * n, n * sizeof(double),
+                                  hipMemcpyDeviceToDevice, D.stream));
+            const rocblas_status s2 = rocsolver_dsteqr(
+                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
+                E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
+            KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dsteqr failed");
+            read_info();
+            KV_REQUIRE(hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), (int64_t)nn)),
+                       ERR_SOLVER, "Hermitian eigensolver did not converge");
+        }
         for (int b = 0; b < nb; ++b) {
             xs[b]->tau = DevBuf(D.pool.get(), (size_t)n * sizeof(double2));
             KV_HIP(hipMemcpyAsync(xs[b]->tau.get(), taub.as<double2>() + (size_t)b * n,
                                   (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
-        }
-    }
-    if (nb > 1)
-        for (int b = 0; b < nb; ++b)
+            // the reflectors, from which gram_eigvecs forms the vectors
             KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
                                   hipMemcpyDeviceToDevice, D.stream));
+        }
+    }
     std::vector<double> w((size_t)n * nb);
-    std::vector<rocblas_int> hinfo(nb);
     KV_HIP(hipMemcpyAsync(w.data(), W.get(), w.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
-    KV_HIP(hipMemcpyAsync(hinfo.data(), info.get(), nb * sizeof(rocblas_int), hipMemcpyDeviceToHost,
-                          D.stream));
     D.sync();
     for (int b = 0; b < nb; ++b) {
-        KV_REQUIRE(hinfo[b] == 0, ERR_SOLVER, "Hermitian eigensolver did not converge");
         SectorSvd& X = *xs[b];
         X.s.resize(n);
-        for (int j = 0; j < n; ++j)
-            X.s[j] = std::sqrt(std::max(w[(size_t)b * n + n - 1 - j], 0.0) * X.gscale);
+        for (int j = 0; j < n; ++j) {
+            const double v = w[(size_t)b * n + n - 1 - j];
+            KV_REQUIRE(std::isfinite(v), ERR_SOLVER, "Hermitian eigensolver returned a non-finite value");
+            X.s[j] = std::sqrt(std::max(v, 0.0) * X.gscale);
+        }
         if (split) {
             X.Ev = DevBuf();
             X.ev_lo = n;  // no vectors formed yet
