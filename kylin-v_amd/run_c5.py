@@ -4,9 +4,11 @@ J = 1, one exciton, two-site TDVP at dt = 0.02 with bond cap 512, cutoff 1e-10, 
 physical start (exciton on site 0, phonon vacuum).  The same steps run with the H_eff
 contractions in FP64, FP32 and split-BF16 (FP64 Lanczos, expm, SVD and environments);
 prints one JSON line with C(t) of the FP64 run, max|C_p(t) - C_fp64(t)| per precision,
-the bond dimensions reached and the wall time per step.
+the bond dimensions reached and the wall time per step.  With --budget-s the first
+precision stops once it has used that much wall time, and the others run the same number
+of steps.
 
-    python kylin-v_amd/run_c5.py [--steps 10] [--N 64]
+    python kylin-v_amd/run_c5.py [--steps 10] [--N 64] [--budget-s 300]
 """
 import argparse
 import json
@@ -26,6 +28,7 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--dt", type=float, default=0.02)
     ap.add_argument("--precisions", default="fp64,fp32,bf16x3")
+    ap.add_argument("--budget-s", type=float, default=0.0)
     args = ap.parse_args()
     import numpy as np
     import __graft_entry__ as GE
@@ -41,18 +44,23 @@ def main():
                        f"D<={args.D}, cutoff 1e-10, physical start",
            "steps": args.steps, "precisions": {}}
     ct = {}
-    for prec in args.precisions.split(","):
+    steps = args.steps
+    for pi, prec in enumerate(args.precisions.split(",")):
         eng = kv.TDVP(ch.psi0, ch.mpo, cutoff=1e-10, maxdim=args.D, dt=args.dt, precision=prec)
         cs = []
         t0 = time.perf_counter()
-        for k in range(args.steps):
+        for k in range(steps):
             eng.two_site()
             cs.append(kv.overlap_states(ch.psi0, eng.mps))
             log(f"{prec} step {k + 1}: C = {cs[-1]:.10f}, max bond {max(eng.bond_dims())}")
+            if pi == 0 and args.budget_s > 0 and time.perf_counter() - t0 > args.budget_s:
+                break
+        steps = len(cs)
+        out["steps"] = steps
         wall = time.perf_counter() - t0
         ct[prec] = np.array(cs)
         st = eng.stats()
-        out["precisions"][prec] = {"s_per_step": wall / args.steps, "max_bond": max(eng.bond_dims()),
+        out["precisions"][prec] = {"s_per_step": wall / steps, "max_bond": max(eng.bond_dims()),
                                    "heff_tflops": st["heff_flop"] / max(st["heff_gpu_s"], 1e-30) / 1e12,
                                    "modules_s": {k: st[k] for k in ("Evl-2", "Evl-1", "SVD", "Update")}}
         del eng
