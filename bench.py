@@ -27,7 +27,7 @@ PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s (AMD published
 # two rocprofv3 --pmc passes over `bench.py --steps 1 --no-cpu --physical-steps 0`
 # (FETCH_SIZE, WRITE_SIZE; mean over 1096 dispatches), each scaled by what the counter
 # reads for f64_to_f32_kernel moving a known 16+8 bytes per element (FETCH_SIZE x2.0,
-# WRITE_SIZE x1.0).  Summary and commands: profiles/c2_merged_D256_gemm_f32_hbm_traffic.md
+# WRITE_SIZE x1.0).  Summary and commands: profiles/chain32_merged_D256_gemm_f32_hbm_traffic.md
 GEMM_F32_TRAFFIC_BYTES = 0.414e9
 N, L, OMEGA, J, DT, D, CUTOFF = 32, 16, 0.5, 1.0, 0.02, 256, 1e-10
 

@@ -18,7 +18,9 @@ import time
 import numpy as np
 
 
-def c3_chains(n=64, seed=3):
+def seeded_chains(n=64, seed=3):
+    """The n chain specs of the batching workload, drawn in a fixed order from
+    default_rng(seed)."""
     rng = np.random.default_rng(seed)
     out = []
     for c in range(n):

@@ -6,7 +6,7 @@ differences (h = 1e-4) of the same device forward on the parameters listed in --
 The start is the exciton-on-site-0 product state expanded to D = 128 (direct sum with a
 seeded random MPS of weight eps, right-canonical, full-rank Schmidt spectra).
 
-    python kylin-v_amd/run_c4.py [--steps 30] [--fd 0,11,23,24,25] [--nodes 4]
+    python kylin-v_amd/run_adjoint.py [--steps 30] [--fd 0,11,23,24,25] [--nodes 4]
 """
 import argparse
 import json
@@ -47,7 +47,7 @@ def main():
                            A.DeviceOps(kv.default_context()), nodes=args.nodes)
 
     def log(msg):
-        print(f"c4: {msg}", file=sys.stderr, flush=True)
+        print(f"adjoint: {msg}", file=sys.stderr, flush=True)
 
     def prog(phase, k):
         log(f"{phase} step {k} done")

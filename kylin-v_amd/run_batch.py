@@ -3,7 +3,7 @@ J ~ U[0.5,1.5], w ~ U[0.3,1.5], g ~ U[0.5,2.0], bond caps U{64..256}, seed 3), t
 TDVP 50 steps at dt = 0.05, advanced concurrently by worker threads with one context
 (HIP stream) each.  Prints one JSON line with chain-steps/s.
 
-    python kylin-v_amd/run_c3.py [--chains 64] [--steps 50] [--workers 8] [--precision fp64]
+    python kylin-v_amd/run_batch.py [--chains 64] [--steps 50] [--workers 8] [--precision fp64]
 """
 import argparse
 import json
@@ -30,7 +30,7 @@ def main():
     GE.load_package()
     from kylin_v_amd import batch
 
-    specs = batch.c3_chains(args.chains, args.seed)
+    specs = batch.seeded_chains(args.chains, args.seed)
     if args.pick:
         idx = [int(x) for x in args.pick.split(",")]
     else:
@@ -49,7 +49,7 @@ def main():
         return free.value / 2**30
 
     def progress(i, r):
-        print(f"c3: chain {idx[i]} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
+        print(f"batch: chain {idx[i]} N={specs[i]['N']} L={specs[i]['L']} D={specs[i]['D']} done in "
               f"{r['seconds']:.1f} s, max bond {max(r['bonds'])}, device free {free_gib():.1f} GiB",
               file=sys.stderr, flush=True)
 

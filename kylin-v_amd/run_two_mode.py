@@ -8,7 +8,7 @@ the bond dimensions reached and the wall time per step.  With --budget-s the fir
 precision stops once it has used that much wall time, and the others run the same number
 of steps.
 
-    python kylin-v_amd/run_c5.py [--steps 10] [--N 64] [--budget-s 300]
+    python kylin-v_amd/run_two_mode.py [--steps 10] [--N 64] [--budget-s 300]
 """
 import argparse
 import json
@@ -36,7 +36,7 @@ def main():
     from kylin_v_amd import model as KM
 
     def log(msg):
-        print(f"c5: {msg}", file=sys.stderr, flush=True)
+        print(f"two-mode: {msg}", file=sys.stderr, flush=True)
 
     ch = KM.HolsteinChain(args.N, [0.5, 1.5], 0.7, [args.L, args.L], 1.0, "merged")
     out = {"workload": f"Holstein chain N={args.N}, exciton x 2 modes (w 0.5, 1.5; {args.L} levels "

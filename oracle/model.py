@@ -35,7 +35,7 @@ from .blocksparse import Tensor
 
 __all__ = ["SiteType", "Lattice", "holstein", "build_mpo", "product_state",
            "density_mpo", "hopping_mpo", "identity_mpo", "mpo_to_dense",
-           "mps_to_dense", "ed_hamiltonian", "ed_initial_state", "c1_params", "c2_params"]
+           "mps_to_dense", "ed_hamiltonian", "ed_initial_state", "chain8_params", "chain32_params"]
 
 
 class SiteType:
@@ -388,12 +388,12 @@ def ed_initial_state(N, Ls):
     return v
 
 
-def c1_params():
+def chain8_params():
     """8 sites, exciton (x) one mode at 8 levels, J = w = g = 1."""
     return dict(N=8, omegas=[1.0], gs=np.ones((8, 1)), Ls=[8], J=1.0)
 
 
-def c2_params(seed=2024):
+def chain32_params(seed=2024):
     """32 sites, 16 levels, J=1, w=0.5, g_i ~ U[0.8, 1.2] from numpy default_rng(seed)."""
     rng = np.random.default_rng(seed)
     g = rng.uniform(0.8, 1.2, size=32)

@@ -1,4 +1,4 @@
-"""Writes tests/golden/c1_<layout>.json: the oracle's C(t), exciton populations and bond
+"""Writes tests/golden/chain8_<layout>.json: the oracle's C(t), exciton populations and bond
 dimensions for the 8-site Holstein chain (exciton x one mode at 8 levels, J = w = g = 1,
 two-site TDVP dt = 0.05, 40 steps, D <= 32, cutoff 1e-10, Krylov tol 1e-8, 15 iterations).
 
@@ -6,7 +6,7 @@ The oracle (oracle/) is the numpy restatement of the reference's algorithm, pinn
 tests/test_oracle.py; these files let the GPU test compare the whole 40-step run without
 re-running the 40-step oracle trajectory inside the GPU test.
 
-    python tests/golden/make_c1_fixture.py [merged|pp ...]
+    python tests/golden/make_chain8_fixture.py [merged|pp ...]
 """
 import json
 import os
@@ -20,7 +20,7 @@ from oracle import tdvp as T  # noqa: E402
 
 
 def run(layout, steps=40):
-    p = M.c1_params()
+    p = M.chain8_params()
     lat = M.holstein(p["N"], p["omegas"], p["gs"], p["Ls"], p["J"], layout)
     mpo = M.build_mpo(lat)
     psi0 = M.product_state(lat)
@@ -44,5 +44,5 @@ def run(layout, steps=40):
 if __name__ == "__main__":
     for layout in (sys.argv[1:] or ["merged", "pp"]):
         d = run(layout)
-        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), f"c1_{layout}.json"), "w") as f:
+        with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), f"chain8_{layout}.json"), "w") as f:
             json.dump(d, f)

@@ -1,6 +1,6 @@
 """The 8-site Holstein chain (one mode at 8 levels, J = w = g = 1, 2TDVP dt = 0.05, 40
 steps, D <= 32, cutoff 1e-10) on the GPU against the oracle's fixtures
-(tests/golden/c1_<layout>.json, written by tests/golden/make_c1_fixture.py)."""
+(tests/golden/chain8_<layout>.json, written by tests/golden/make_chain8_fixture.py)."""
 import json
 import os
 import sys
@@ -17,11 +17,11 @@ pytestmark = pytest.mark.gpu
 
 @pytest.mark.parametrize("layout", ["merged", "pp"])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_c1_run_matches_oracle_fixture(layout, precision):
-    print(f"case c1 {layout} {precision}", flush=True)
+def test_chain8_run_matches_oracle_fixture(layout, precision):
+    print(f"case chain8 {layout} {precision}", flush=True)
     kv = GE.load_package()
     from kylin_v_amd import model as KM
-    ref = json.load(open(os.path.join(ROOT, "tests", "golden", f"c1_{layout}.json")))
+    ref = json.load(open(os.path.join(ROOT, "tests", "golden", f"chain8_{layout}.json")))
     N = ref["N"]
     ch = KM.HolsteinChain(N, [ref["omega"]], ref["g"], [ref["L"]], ref["J"], layout)
     eng = kv.TDVP(ch.psi0, ch.mpo, cutoff=ref["cutoff"], maxdim=ref["D"], dt=ref["dt"],
@@ -38,7 +38,7 @@ def test_c1_run_matches_oracle_fixture(layout, precision):
     Cref = np.array([complex(a, b) for a, b in ref["C"]])
     err_c = np.abs(C - Cref).max()
     err_p = np.abs(np.array(pops) - np.array(ref["pops"])).max()
-    print(f"c1 {layout} {precision}: max|dC| = {err_c:.3e}, max|dP| = {err_p:.3e}", flush=True)
+    print(f"chain8 {layout} {precision}: max|dC| = {err_c:.3e}, max|dP| = {err_p:.3e}", flush=True)
     tol = 1e-8 if precision == "fp64" else 1e-6
     assert err_c < tol and err_p < tol
     assert abs(np.array(pops[-1]).sum() - 1.0) < 1e-8
