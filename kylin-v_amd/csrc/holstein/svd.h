@@ -220,15 +220,6 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
 // Reflectors per block of the back-transformation.
 constexpr int kReflectorBlock = 128;
 
-// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n.  The
-// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
-// eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
-// Split solver (by default n >= kSplitEigMin): one batched zhetrd for all sectors (its
-// tridiagonalisation is launch-latency bound, so batching sectors costs about as much as
-// one), then dstedc per sector; the vectors are formed later, only for the rows the
-// truncation needs (gram_eigvecs).  zheevd (all vectors at once, batched) otherwise.  (dstedc of a pair on two streams measured slower in
-// the sweep than one after the other: the cross-stream event waits cost more than the
-// overlap.)
 // Sum of squares of `count` doubles on the device (NaN or Inf if any is).
 inline double dev_sumsq(Device& D, const double* p, int64_t count) {
     DevBuf sc(D.pool.get(), 2048 * sizeof(double)), out(D.pool.get(), sizeof(double));
@@ -242,6 +233,15 @@ inline double dev_sumsq(Device& D, const double* p, int64_t count) {
     return v + tail * tail;
 }
 
+// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n.  The
+// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
+// eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
+// Split solver (by default n >= kSplitEigMin): one batched zhetrd for all sectors (its
+// tridiagonalisation is launch-latency bound, so batching sectors costs about as much as
+// one), then dstedc per sector; the vectors are formed later, only for the rows the
+// truncation needs (gram_eigvecs).  zheevd (all vectors at once, batched) otherwise.
+// (dstedc of a pair on two streams measured slower in the sweep than one after the
+// other: the cross-stream event waits cost more than the overlap.)
 inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO) {
     const int n = xs[0]->k;
     const int nb = (int)xs.size();
