@@ -233,28 +233,50 @@ inline double dev_sumsq(Device& D, const double* p, int64_t count) {
     return v + tail * tail;
 }
 
-// Eigen-decomposition of the Gram matrices of one or more sectors of equal size n.  The
-// row-major Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
+// Eigen-decomposition of the Gram matrices of one or more sectors.  The row-major
+// Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
 // eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
-// Split solver (by default n >= kSplitEigMin): one batched zhetrd for all sectors (its
-// tridiagonalisation is launch-latency bound, so batching sectors costs about as much as
-// one), then dstedc per sector; the vectors are formed later, only for the rows the
-// truncation needs (gram_eigvecs).  zheevd (all vectors at once, batched) otherwise.
-// (dstedc of a pair on two streams measured slower in the sweep than one after the
-// other: the cross-stream event waits cost more than the overlap.)
+// Tridiagonal route (sides >= kSplitEigMin by default, or sectors of different sides): one
+// batched zhetrd for all sectors (its cost is per column, not per matrix, so the sectors of
+// a bond cost about as much as the largest), every sector padded to the largest side N as
+// diag(G, -I) (G is positive semidefinite, so the padding stays exactly decoupled and each
+// sector's reflectors and tridiagonal are those of G); then dstedc per sector at its own
+// side, and the vectors formed later, only for the rows the truncation needs
+// (gram_eigvecs).  zheevd (all vectors at once, batched) for equal sides below
+// kSplitEigMin.  (zheevd cannot take the padded matrices: on the split tridiagonal its
+// divide and conquer returns NaN eigenvectors.  dstedc of a pair on two streams measured
+// slower in the sweep than one after the other.)
 inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO) {
-    const int n = xs[0]->k;
     const int nb = (int)xs.size();
-    const size_t nn = (size_t)n * n;
-    const bool split = mode == EIG_SPLIT ? n >= 2 : mode == EIG_AUTO && n >= kSplitEigMin;
-    DevBuf W(D.pool.get(), (size_t)n * nb * sizeof(double)), E(D.pool.get(), (size_t)n * nb * sizeof(double));
+    int N = 0;
+    bool same = true;
+    for (auto* x : xs) {
+        same = same && (N == 0 || x->k == N);
+        N = std::max(N, x->k);
+    }
+    const size_t NN = (size_t)N * N;
+    const bool split = mode == EIG_SPLIT ? N >= 2
+                       : mode == EIG_AUTO ? (N >= kSplitEigMin || (!same && N >= 2))
+                                          : false;
+    KV_REQUIRE(split || same, ERR_INTERNAL, "zheevd needs sectors of one side");
+    DevBuf W(D.pool.get(), (size_t)N * nb * sizeof(double)), E(D.pool.get(), (size_t)N * nb * sizeof(double));
     DevBuf info(D.pool.get(), sizeof(rocblas_int) * nb);
     // the solvers overwrite their input: they work on copies, so that a sector the divide
     // and conquer fails on can be redone from its Gram matrix (or tridiagonal form)
-    DevBuf Gb(D.pool.get(), nn * nb * sizeof(double2)), taub;
-    for (int b = 0; b < nb; ++b)
-        KV_HIP(hipMemcpyAsync(Gb.as<double2>() + b * nn, xs[b]->G.get(), nn * sizeof(double2),
-                              hipMemcpyDeviceToDevice, D.stream));
+    DevBuf Gb(D.pool.get(), NN * nb * sizeof(double2)), taub;
+    for (int b = 0; b < nb; ++b) {
+        double2* slot = Gb.as<double2>() + b * NN;
+        const int n = xs[b]->k;
+        if (n == N) {
+            KV_HIP(hipMemcpyAsync(slot, xs[b]->G.get(), NN * sizeof(double2), hipMemcpyDeviceToDevice,
+                                  D.stream));
+        } else {
+            launch_zero(D.stream, slot, NN * sizeof(double2));
+            KV_HIP(hipMemcpy2DAsync(slot, (size_t)N * sizeof(double2), xs[b]->G.get(), (size_t)n * sizeof(double2),
+                                    (size_t)n * sizeof(double2), n, hipMemcpyDeviceToDevice, D.stream));
+            launch_set_diag(D.stream, slot, N, n, N, -1.0);
+        }
+    }
     auto* gp = reinterpret_cast<rocblas_double_complex*>(Gb.get());
     std::vector<rocblas_int> hinfo(nb);
     auto read_info = [&] {
@@ -267,17 +289,18 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
     // product state): it reports non-convergence, or returns NaN eigenvectors with info 0.
     // Such sectors are redone by implicit QR iteration (zheev / dsteqr).
     if (!split) {
+        const int n = N;
         const rocblas_status st = rocsolver_zheevd_strided_batched(
-            sol.h, rocblas_evect_original, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
+            sol.h, rocblas_evect_original, rocblas_fill_lower, n, gp, n, (rocblas_stride)NN, W.as<double>(), n,
             E.as<double>(), n, info.as<rocblas_int>(), nb);
         KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zheevd failed");
         read_info();
         for (int b = 0; b < nb; ++b) {
-            double2* slot = Gb.as<double2>() + b * nn;
+            double2* slot = Gb.as<double2>() + b * NN;
             const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, reinterpret_cast<const double*>(slot),
-                                                                      2 * (int64_t)nn));
+                                                                      2 * (int64_t)NN));
             if (ok) continue;
-            KV_HIP(hipMemcpyAsync(slot, xs[b]->G.get(), nn * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
+            KV_HIP(hipMemcpyAsync(slot, xs[b]->G.get(), NN * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
             const rocblas_status s2 = rocsolver_zheev(sol.h, rocblas_evect_original, rocblas_fill_lower, n,
                                                       reinterpret_cast<rocblas_double_complex*>(slot), n,
                                                       W.as<double>() + (size_t)b * n, E.as<double>() + (size_t)b * n,
@@ -285,63 +308,69 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_zheev failed");
             read_info();
             KV_REQUIRE(hinfo[b] == 0 && std::isfinite(dev_sumsq(D, reinterpret_cast<const double*>(slot),
-                                                                 2 * (int64_t)nn)),
+                                                                 2 * (int64_t)NN)),
                        ERR_SOLVER, "Hermitian eigensolver did not converge");
         }
         for (int b = 0; b < nb; ++b)
-            KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
+            KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * NN, NN * sizeof(double2),
                                   hipMemcpyDeviceToDevice, D.stream));
     } else {
-        taub = DevBuf(D.pool.get(), (size_t)n * nb * sizeof(double2));
+        taub = DevBuf(D.pool.get(), (size_t)N * nb * sizeof(double2));
         auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
         const rocblas_status st = rocsolver_zhetrd_strided_batched(
-            sol.h, rocblas_fill_lower, n, gp, n, (rocblas_stride)nn, W.as<double>(), n,
-            E.as<double>(), n, tp, n, nb);
+            sol.h, rocblas_fill_lower, N, gp, N, (rocblas_stride)NN, W.as<double>(), N,
+            E.as<double>(), N, tp, N, nb);
         KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
         // the tridiagonal forms, kept for a redo (dstedc overwrites both arrays)
         DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
         KV_HIP(hipMemcpyAsync(W0.get(), W.get(), W.bytes(), hipMemcpyDeviceToDevice, D.stream));
         KV_HIP(hipMemcpyAsync(E0.get(), E.get(), E.bytes(), hipMemcpyDeviceToDevice, D.stream));
         for (int b = 0; b < nb; ++b) {
-            xs[b]->Zt = DevBuf(D.pool.get(), nn * sizeof(double));
+            const int n = xs[b]->k;
+            xs[b]->Zt = DevBuf(D.pool.get(), (size_t)n * n * sizeof(double));
             const rocblas_status s2 = rocsolver_dstedc(
-                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
-                E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
+                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * N,
+                E.as<double>() + (size_t)b * N, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
         read_info();
         for (int b = 0; b < nb; ++b) {
-            const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), (int64_t)nn));
+            const int n = xs[b]->k;
+            const int64_t nn = (int64_t)n * n;
+            const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), nn));
             if (ok) continue;
-            KV_HIP(hipMemcpyAsync(W.as<double>() + (size_t)b * n, W0.as<double>() + (size_t)b * n, n * sizeof(double),
+            KV_HIP(hipMemcpyAsync(W.as<double>() + (size_t)b * N, W0.as<double>() + (size_t)b * N, n * sizeof(double),
                                   hipMemcpyDeviceToDevice, D.stream));
-            KV_HIP(hipMemcpyAsync(E.as<double>() + (size_t)b * n, E0.as<double>() + (size_t)b * n, n * sizeof(double),
+            KV_HIP(hipMemcpyAsync(E.as<double>() + (size_t)b * N, E0.as<double>() + (size_t)b * N, n * sizeof(double),
                                   hipMemcpyDeviceToDevice, D.stream));
             const rocblas_status s2 = rocsolver_dsteqr(
-                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * n,
-                E.as<double>() + (size_t)b * n, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
+                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * N,
+                E.as<double>() + (size_t)b * N, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dsteqr failed");
             read_info();
-            KV_REQUIRE(hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), (int64_t)nn)),
-                       ERR_SOLVER, "Hermitian eigensolver did not converge");
+            KV_REQUIRE(hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), nn)), ERR_SOLVER,
+                       "Hermitian eigensolver did not converge");
         }
         for (int b = 0; b < nb; ++b) {
+            const int n = xs[b]->k;
             xs[b]->tau = DevBuf(D.pool.get(), (size_t)n * sizeof(double2));
-            KV_HIP(hipMemcpyAsync(xs[b]->tau.get(), taub.as<double2>() + (size_t)b * n,
+            KV_HIP(hipMemcpyAsync(xs[b]->tau.get(), taub.as<double2>() + (size_t)b * N,
                                   (size_t)n * sizeof(double2), hipMemcpyDeviceToDevice, D.stream));
-            // the reflectors, from which gram_eigvecs forms the vectors
-            KV_HIP(hipMemcpyAsync(xs[b]->G.get(), Gb.as<double2>() + b * nn, nn * sizeof(double2),
-                                  hipMemcpyDeviceToDevice, D.stream));
+            // the reflectors (the sector's n x n corner), from which gram_eigvecs forms the vectors
+            KV_HIP(hipMemcpy2DAsync(xs[b]->G.get(), (size_t)n * sizeof(double2), Gb.as<double2>() + b * NN,
+                                    (size_t)N * sizeof(double2), (size_t)n * sizeof(double2), n,
+                                    hipMemcpyDeviceToDevice, D.stream));
         }
     }
-    std::vector<double> w((size_t)n * nb);
+    std::vector<double> w((size_t)N * nb);
     KV_HIP(hipMemcpyAsync(w.data(), W.get(), w.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
     D.sync();
     for (int b = 0; b < nb; ++b) {
         SectorSvd& X = *xs[b];
+        const int n = X.k;
         X.s.resize(n);
         for (int j = 0; j < n; ++j) {
-            const double v = w[(size_t)b * n + n - 1 - j];
+            const double v = w[(size_t)b * N + n - 1 - j];
             KV_REQUIRE(std::isfinite(v), ERR_SOLVER, "Hermitian eigensolver returned a non-finite value");
             X.s[j] = std::sqrt(std::max(v, 0.0) * X.gscale);
         }
@@ -675,12 +704,19 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         else
             exact_sector(D, sol, X[p], cutoff < 0.0);
     }
-    // Gram eigenproblems, batched by size
+    // Gram eigenproblems: one call for all sectors (zheevd needs one side per call)
     {
         std::map<int, std::vector<SectorSvd*>> by_n;
+        std::vector<SectorSvd*> all;
         for (int p = 0; p < np; ++p)
-            if (X[p].gram) by_n[X[p].k].push_back(&X[p]);
-        for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_mode);
+            if (X[p].gram) {
+                by_n[X[p].k].push_back(&X[p]);
+                all.push_back(&X[p]);
+            }
+        if (opt.eig_mode == EIG_HEEVD || by_n.size() <= 1)
+            for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_mode);
+        else
+            gram_solve(D, sol, all, opt.eig_mode);
         // the isometric route's values: only min(rows, cols) are singular values
         for (int p = 0; p < np; ++p)
             if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));

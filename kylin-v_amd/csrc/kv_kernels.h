@@ -63,6 +63,9 @@ void launch_f64_to_f32(hipStream_t st, const double2* src, float2* dst, int64_t 
 // dst = s * src
 void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s, int64_t n);
 
+// A[i ld + i] = (v, 0) for i in [from, to) (the diagonal of a square matrix).
+void launch_set_diag(hipStream_t st, double2* A, int64_t ld, int from, int to, double v);
+
 // Back-transformation of a lower-triangle Householder tridiagonalisation (LAPACK layout:
 // reflector i stored below the subdiagonal of column i, tau_i separately).
 // V (column-major n x ncols): the reflector vectors written out with their unit entries.

@@ -931,6 +931,19 @@ void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s
     KV_HIP(hipGetLastError());
 }
 
+// A[i ld + i] = (v, 0) for i in [from, to): the diagonal of a square matrix (either order).
+__global__ __launch_bounds__(256) void set_diag_kernel(double2* __restrict__ A, int64_t ld, int from,
+                                                       int to, double v) {
+    for (int i = from + blockIdx.x * 256 + threadIdx.x; i < to; i += gridDim.x * 256)
+        A[(int64_t)i * ld + i] = make_double2(v, 0.0);
+}
+
+void launch_set_diag(hipStream_t st, double2* A, int64_t ld, int from, int to, double v) {
+    if (to <= from) return;
+    hipLaunchKernelGGL(set_diag_kernel, dim3(ew_blocks(to - from)), dim3(256), 0, st, A, ld, from, to, v);
+    KV_HIP(hipGetLastError());
+}
+
 // ---- back-transformation of a Householder tridiagonalisation ------------------------
 // Lower-triangle reduction Q^H A Q = T of a column-major n x n matrix: reflector i < n-1 is
 // H_i = I - tau_i v_i v_i^H with v_i[r] = 0 for r <= i, 1 at r = i + 1 and A[i n + r] below.
