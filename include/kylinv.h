@@ -160,10 +160,12 @@ int kv_krylov_step(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* 
                    int reverse, double tol, int max_iter, int precision,
                    kv_krylov_exit* exit_info);
 
-/* svd_impl<R1,R2>(A, cfg, LeftDir=-1) (holstein/svd.h:296-440): sector SVD in FP64
- * (rocSOLVER zgesvd), global descending sort, drop entry i (never 0) while
- * i >= maxdim or s_i <= cutoff; S absorbed into V^dag if left_to_right else into U.
- * nkept receives the kept bond dimension. */
+/* svd_impl<R1,R2>(A, cfg, LeftDir=-1) (holstein/svd.h:296-440): sector SVD in FP64,
+ * global descending sort, drop entry i (never 0) while i >= maxdim or s_i <= cutoff;
+ * S absorbed into V^dag if left_to_right else into U.  nkept receives the kept bond
+ * dimension.  Sectors whose smaller side is at least 64 are solved through their Gram
+ * matrix on the isometric side (rocSOLVER Hermitian eigensolvers, a second pass over the
+ * tail below 1e-5 of the largest value); smaller ones by rocSOLVER's Jacobi SVD. */
 int kv_svd_truncate(kv_ctx* ctx, const kv_tensor* A, int r1, double cutoff, int maxdim,
                     int left_to_right, kv_tensor** left, kv_tensor** right, int* nkept);
 
