@@ -15,8 +15,9 @@
  *   left env   (a', w, a)       dirs {+1,-1,-1}, boundary qn (0,0,0)
  *   right env  (b', w, b)       dirs {-1,+1,+1}, boundary qn (Q,0,Q)
  *                                             (holstein/tdvp.h:302-311)
- * Every call is ordered on the context's HIP stream; calls that return host values
- * synchronise that stream.  One context per host thread.
+ * Every call runs on the context's HIP stream and returns when that work has finished
+ * (the reference's flow reads every result on the host before the next call).  One
+ * context per host thread; contexts on one device may run concurrently.
  */
 #ifndef KYLINV_H
 #define KYLINV_H
@@ -35,7 +36,8 @@ typedef struct kv_tdvp kv_tdvp;
  * scalars, vector algebra, the Krylov exponential and the bond SVD are FP64 always. */
 enum kv_precision {
     KV_FP64 = 0,   /* complex FP64 on v_mfma_f64_16x16x4f64                        */
-    KV_FP32 = 1,   /* complex FP32 on v_mfma_f32_16x16x4f32, FP32 accumulation     */
+    KV_FP32 = 1,   /* complex FP32 on v_mfma_f32_16x16x4f32 (three real products per
+                      complex one), FP32 accumulation                               */
     KV_BF16X3 = 2  /* FP32 storage; H_eff GEMMs as split-BF16 products (hi*hi + hi*lo +
                       lo*hi) on v_mfma_f32_16x16x32_bf16, FP32 accumulation; environment
                       updates requested in this mode run their GEMMs in FP32              */
