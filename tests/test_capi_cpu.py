@@ -35,3 +35,19 @@ def test_parse_stage_matches_reference_format():
     kv = GE.load_package()
     assert kv.parse_stage("[0,40,0,0]") == [0, 40, 0, 0]
     assert kv.parse_stage("[3,50,2447,0]") == [3, 50, 2447, 0]
+
+
+def test_header_is_plain_c(tmp_path):
+    """include/kylinv.h compiles as strict C99 and as C++ (the boundary carries no C++ or
+    HIP types)."""
+    import shutil
+    import subprocess
+    src = tmp_path / "use.c"
+    src.write_text('#include "kylinv.h"\nint main(void) { kv_ctx* c = 0; (void)c; return 0; }\n')
+    inc = os.path.join(ROOT, "include")
+    if shutil.which("gcc"):
+        subprocess.run(["gcc", "-std=c99", "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only",
+                        "-I", inc, str(src)], check=True)
+    if shutil.which("g++"):
+        subprocess.run(["g++", "-std=c++17", "-Wall", "-Werror", "-fsyntax-only", "-x", "c++", "-I", inc,
+                        str(src)], check=True)
