@@ -355,85 +355,121 @@ __global__ __launch_bounds__(64 * (BM / (16 * SI)) * (BN / (16 * SJ))) void zgem
             cim[i][j] = f32x4{0, 0, 0, 0};
         }
 
-    for (int s = 0; s < g.seg_count; ++s) {
+    // K stages run over (segment, k0) pairs.  The raw FP32 operands of the next stage are
+    // loaded into registers while the current one is multiplied out of LDS; they are split
+    // into the bf16 planes and stored once every wave has finished reading.
+    constexpr int RA = (BM * BK / 2) / NT, RB = (BN * BK / 2) / NT;
+    float2 ra[RA][2], rb[RB][2];
+    auto load = [&](int s, int k0) {
         const GemmSeg sg = segs[g.seg_begin + s];
         const float2* __restrict__ a = A + sg.a_off;
         const float2* __restrict__ b = B + sg.b_off;
         const int K = sg.k;
-        for (int k0 = 0; k0 < K; k0 += BK) {
-            // stage A: pairs of consecutive K per thread
 #pragma unroll
-            for (int r = 0; r < (BM * BK / 2) / NT; ++r) {
-                const int e = tid + r * NT;
-                int mm, kk;
+        for (int r = 0; r < RA; ++r) {
+            const int e = tid + r * NT;
+            int mm, kk;
+            if (OPA == 0) {
+                mm = e / (BK / 2);
+                kk = 2 * (e % (BK / 2));
+            } else {
+                mm = e % BM;
+                kk = 2 * (e / BM);
+            }
+            const int gm = m0 + mm, gk = k0 + kk;
+            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+            if (gm < g.m) {
                 if (OPA == 0) {
-                    mm = e / (BK / 2);
-                    kk = 2 * (e % (BK / 2));
+                    const float2* row = a + (int64_t)gm * sg.lda;
+                    if (gk < K) v0 = row[gk];
+                    if (gk + 1 < K) v1 = row[gk + 1];
                 } else {
-                    mm = e % BM;
-                    kk = 2 * (e / BM);
+                    if (gk < K) v0 = a[(int64_t)gk * sg.lda + gm];
+                    if (gk + 1 < K) v1 = a[(int64_t)(gk + 1) * sg.lda + gm];
                 }
-                const int gm = m0 + mm, gk = k0 + kk;
-                float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
-                if (gm < g.m) {
-                    if (OPA == 0) {
-                        const float2* row = a + (int64_t)gm * sg.lda;
-                        if (gk < K) v0 = row[gk];
-                        if (gk + 1 < K) v1 = row[gk + 1];
-                    } else {
-                        if (gk < K) v0 = a[(int64_t)gk * sg.lda + gm];
-                        if (gk + 1 < K) v1 = a[(int64_t)(gk + 1) * sg.lda + gm];
-                    }
-                }
-                if (OPA == 2) {
-                    v0.y = -v0.y;
-                    v1.y = -v1.y;
-                }
-                bf16x2 h, l;
-                split2(v0.x, v1.x, h, l);
-                *(bf16x2*)&As[0][mm][kk] = h;
-                *(bf16x2*)&As[1][mm][kk] = l;
-                split2(v0.y, v1.y, h, l);
-                *(bf16x2*)&As[2][mm][kk] = h;
-                *(bf16x2*)&As[3][mm][kk] = l;
             }
-            // stage B
+            if (OPA == 2) {
+                v0.y = -v0.y;
+                v1.y = -v1.y;
+            }
+            ra[r][0] = v0;
+            ra[r][1] = v1;
+        }
 #pragma unroll
-            for (int r = 0; r < (BN * BK / 2) / NT; ++r) {
-                const int e = tid + r * NT;
-                int nn, kk;
-                if (OPB == 0) {
-                    nn = e % BN;
-                    kk = 2 * (e / BN);
-                } else {
-                    nn = e / (BK / 2);
-                    kk = 2 * (e % (BK / 2));
-                }
-                const int gn = n0 + nn, gk = k0 + kk;
-                float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
-                if (gn < g.n) {
-                    if (OPB == 0) {
-                        if (gk < K) v0 = b[(int64_t)gk * sg.ldb + gn];
-                        if (gk + 1 < K) v1 = b[(int64_t)(gk + 1) * sg.ldb + gn];
-                    } else {
-                        const float2* col = b + (int64_t)gn * sg.ldb;
-                        if (gk < K) v0 = col[gk];
-                        if (gk + 1 < K) v1 = col[gk + 1];
-                    }
-                }
-                if (OPB == 2) {
-                    v0.y = -v0.y;
-                    v1.y = -v1.y;
-                }
-                bf16x2 h, l;
-                split2(v0.x, v1.x, h, l);
-                *(bf16x2*)&Bs[0][nn][kk] = h;
-                *(bf16x2*)&Bs[1][nn][kk] = l;
-                split2(v0.y, v1.y, h, l);
-                *(bf16x2*)&Bs[2][nn][kk] = h;
-                *(bf16x2*)&Bs[3][nn][kk] = l;
+        for (int r = 0; r < RB; ++r) {
+            const int e = tid + r * NT;
+            int nn, kk;
+            if (OPB == 0) {
+                nn = e % BN;
+                kk = 2 * (e / BN);
+            } else {
+                nn = e / (BK / 2);
+                kk = 2 * (e % (BK / 2));
             }
-            __syncthreads();
+            const int gn = n0 + nn, gk = k0 + kk;
+            float2 v0 = make_float2(0.f, 0.f), v1 = make_float2(0.f, 0.f);
+            if (gn < g.n) {
+                if (OPB == 0) {
+                    if (gk < K) v0 = b[(int64_t)gk * sg.ldb + gn];
+                    if (gk + 1 < K) v1 = b[(int64_t)(gk + 1) * sg.ldb + gn];
+                } else {
+                    const float2* col = b + (int64_t)gn * sg.ldb;
+                    if (gk < K) v0 = col[gk];
+                    if (gk + 1 < K) v1 = col[gk + 1];
+                }
+            }
+            if (OPB == 2) {
+                v0.y = -v0.y;
+                v1.y = -v1.y;
+            }
+            rb[r][0] = v0;
+            rb[r][1] = v1;
+        }
+    };
+    auto store = [&] {
+#pragma unroll
+        for (int r = 0; r < RA; ++r) {
+            const int e = tid + r * NT;
+            const int mm = (OPA == 0) ? e / (BK / 2) : e % BM;
+            const int kk = (OPA == 0) ? 2 * (e % (BK / 2)) : 2 * (e / BM);
+            bf16x2 h, l;
+            split2(ra[r][0].x, ra[r][1].x, h, l);
+            *(bf16x2*)&As[0][mm][kk] = h;
+            *(bf16x2*)&As[1][mm][kk] = l;
+            split2(ra[r][0].y, ra[r][1].y, h, l);
+            *(bf16x2*)&As[2][mm][kk] = h;
+            *(bf16x2*)&As[3][mm][kk] = l;
+        }
+#pragma unroll
+        for (int r = 0; r < RB; ++r) {
+            const int e = tid + r * NT;
+            const int nn = (OPB == 0) ? e % BN : e / (BK / 2);
+            const int kk = (OPB == 0) ? 2 * (e / BN) : 2 * (e % (BK / 2));
+            bf16x2 h, l;
+            split2(rb[r][0].x, rb[r][1].x, h, l);
+            *(bf16x2*)&Bs[0][nn][kk] = h;
+            *(bf16x2*)&Bs[1][nn][kk] = l;
+            split2(rb[r][0].y, rb[r][1].y, h, l);
+            *(bf16x2*)&Bs[2][nn][kk] = h;
+            *(bf16x2*)&Bs[3][nn][kk] = l;
+        }
+    };
+    int s = 0, k0 = 0;
+    while (s < g.seg_count && segs[g.seg_begin + s].k <= 0) ++s;
+    if (s < g.seg_count) {
+        load(s, 0);
+        store();
+    }
+    __syncthreads();
+    while (s < g.seg_count) {
+        int s2 = s, k2 = k0 + BK;
+        if (k2 >= segs[g.seg_begin + s].k) {
+            k2 = 0;
+            ++s2;
+            while (s2 < g.seg_count && segs[g.seg_begin + s2].k <= 0) ++s2;
+        }
+        if (s2 < g.seg_count) load(s2, k2);
+        {
             bf16x8 af[SI][4], bf[SJ][4], bn[SJ][2];
 #pragma unroll
             for (int i = 0; i < SI; ++i)
@@ -468,8 +504,14 @@ __global__ __launch_bounds__(64 * (BM / (16 * SI)) * (BN / (16 * SJ))) void zgem
                     cre[i][j] = re;
                     cim[i][j] = im;
                 }
+        }
+        __syncthreads();  // every wave has read this stage's planes
+        if (s2 < g.seg_count) {
+            store();
             __syncthreads();
         }
+        s = s2;
+        k0 = k2;
     }
 #pragma unroll
     for (int i = 0; i < SI; ++i)
