@@ -90,8 +90,10 @@ struct SvdOptions {
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
     int eig_mode = EIG_AUTO;  // Gram eigensolver: split from kSplitEigMin, or forced
+    bool roc_trd = false;     // split route: rocSOLVER zhetrd instead of hetrd_lower
     static SvdOptions from_env() {
         SvdOptions o;
+        if (const char* e = std::getenv("KV_TRD")) o.roc_trd = std::string(e) == "rocsolver";
         if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
         if (const char* e = std::getenv("KV_EIG"))
             o.eig_mode = std::string(e) == "heevd" ? EIG_HEEVD : std::string(e) == "split" ? EIG_SPLIT : EIG_AUTO;
@@ -233,6 +235,54 @@ inline double dev_sumsq(Device& D, const double* p, int64_t count) {
     return v + tail * tail;
 }
 
+// Batched Householder tridiagonalisation of `batch` column-major N x N Hermitian matrices
+// at stride N * N, of which the lower triangle is read.  On exit the storage is LAPACK
+// zhetrd('L')'s: d = diag(T), e = subdiag(T), tau and the reflector tails below the
+// subdiagonal (the upper triangle is overwritten).  Per column two launches (trd_column:
+// the column brought up to date by the panel and reflected; trd_matvec: the trailing
+// matrix times the reflector, as column dot products over the fully stored matrix), and per
+// panel of kTrdPanel columns the rank-2 update of the trailing matrix, both triangles, as two
+// rocBLAS GEMMs.  rocSOLVER's one-stage zhetrd spends five launches per column.
+inline void hetrd_lower(Device& D, Solver& sol, double2* A, int N, int batch, double* d, double* e,
+                        double2* tau) {
+    const int nb = kTrdPanel;
+    DevBuf VW(D.pool.get(), (size_t)N * 2 * nb * batch * sizeof(double2));
+    DevBuf y(D.pool.get(), (size_t)N * batch * sizeof(double2));
+    DevBuf q(D.pool.get(), (size_t)2 * nb * batch * sizeof(double2));
+    DevBuf part(D.pool.get(), (size_t)trd_column_blocks(N, 0) * batch * sizeof(double2));
+    DevBuf cnt(D.pool.get(), (size_t)batch * sizeof(unsigned));
+    launch_zero(D.stream, cnt.get(), cnt.bytes());
+    launch_herm_fill(D.stream, A, N, batch);
+    double2* vw = VW.as<double2>();
+    auto* ac = reinterpret_cast<rocblas_double_complex*>(A);
+    auto* vc = reinterpret_cast<rocblas_double_complex*>(vw);
+    const rocblas_double_complex one(1.0, 0.0), mone(-1.0, 0.0);
+    const rocblas_stride sp = (rocblas_stride)N * 2 * nb, sa = (rocblas_stride)N * N;
+    for (int p = 0; p < N; p += nb) {
+        const int pw = std::min(nb, N - p);
+        for (int i = 0; i < pw; ++i) {
+            const int j = p + i;
+            launch_trd_column(D.stream, A, vw, y.as<double2>(), q.as<double2>(), d, e, tau, part.as<double2>(),
+                              cnt.as<unsigned>(), N, j, i, nb, i > 0, true, batch);
+            if (j <= N - 2)
+                launch_trd_matvec(D.stream, A, vw, y.as<double2>(), q.as<double2>(), N, j + 1, i, nb, batch);
+        }
+        const int t = p + pw, m = N - t;
+        if (m <= 0) break;  // the panel ended on the last column
+        // the last column's companion, then A(t:, t:) -= V W^H + W V^H
+        launch_trd_column(D.stream, A, vw, y.as<double2>(), q.as<double2>(), d, e, tau, part.as<double2>(),
+                          cnt.as<unsigned>(), N, t, pw, nb, true, false, batch);
+        for (int h = 0; h < 2; ++h) {
+            const rocblas_double_complex* a = vc + (h ? (size_t)nb * N : 0) + t;
+            const rocblas_double_complex* b = vc + (h ? 0 : (size_t)nb * N) + t;
+            const rocblas_status st = rocblas_zgemm_strided_batched(
+                sol.h, rocblas_operation_none, rocblas_operation_conjugate_transpose, m, m, pw, &mone, a, N, sp, b,
+                N, sp, &one, ac + (size_t)t * N + t, N, sa, batch);
+            KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocBLAS zgemm failed");
+        }
+    }
+}
+
 // Eigen-decomposition of the Gram matrices of one or more sectors.  The row-major
 // Hermitian buffer read column-major is conj(G): eigenvalues are equal and the
 // eigenvectors are conj(v_j), so row j - ev_lo of Ev read row-major is v_j^H (ascending).
@@ -246,7 +296,8 @@ inline double dev_sumsq(Device& D, const double* p, int64_t count) {
 // kSplitEigMin.  (zheevd cannot take the padded matrices: on the split tridiagonal its
 // divide and conquer returns NaN eigenvectors.  dstedc of a pair on two streams measured
 // slower in the sweep than one after the other.)
-inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO) {
+inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO,
+                       bool roc_trd = false) {
     const int nb = (int)xs.size();
     int N = 0;
     bool same = true;
@@ -316,11 +367,15 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                                   hipMemcpyDeviceToDevice, D.stream));
     } else {
         taub = DevBuf(D.pool.get(), (size_t)N * nb * sizeof(double2));
-        auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
-        const rocblas_status st = rocsolver_zhetrd_strided_batched(
-            sol.h, rocblas_fill_lower, N, gp, N, (rocblas_stride)NN, W.as<double>(), N,
-            E.as<double>(), N, tp, N, nb);
-        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
+        if (roc_trd) {
+            auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
+            const rocblas_status st = rocsolver_zhetrd_strided_batched(
+                sol.h, rocblas_fill_lower, N, gp, N, (rocblas_stride)NN, W.as<double>(), N,
+                E.as<double>(), N, tp, N, nb);
+            KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
+        } else {
+            hetrd_lower(D, sol, Gb.as<double2>(), N, nb, W.as<double>(), E.as<double>(), taub.as<double2>());
+        }
         // the tridiagonal forms, kept for a redo (dstedc overwrites both arrays)
         DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
         KV_HIP(hipMemcpyAsync(W0.get(), W.get(), W.bytes(), hipMemcpyDeviceToDevice, D.stream));
@@ -560,7 +615,7 @@ inline void gram_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
 // tail * s_max (buffer rows [0, nt) in ascending order) span an accurate subspace; M
 // restricted to it (Mt = Bt M or M Bt^H) has norm ~ tail * s_max, and the eigenvectors Y
 // of its own Gram matrix rotate the tail rows: Bt <- Y^H Bt.
-inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, int mode) {
+inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, int mode, bool roc_trd) {
     X.refined = true;
     const int n = X.k;
     const int kmax = (int)std::min(X.nr, X.nc);
@@ -597,7 +652,7 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, int mo
         gemm_dense(D, 'C', 'N', Mt.as<double2>(), nt, Mt.as<double2>(), nt, T.G.as<double2>(),
                    nt, nt, nt, (int)X.nr);
     }
-    gram_solve(D, sol, {&T}, mode);
+    gram_solve(D, sol, {&T}, mode, roc_trd);
     gram_eigvecs(D, sol, T, 0);
     for (double& v : T.s) v *= (ref > 0 ? ref : 1.0);
     // Bt <- Ybuf Bt (rows y_j^H, ascending) and the tail values replaced
@@ -714,9 +769,9 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
                 all.push_back(&X[p]);
             }
         if (opt.eig_mode == EIG_HEEVD || by_n.size() <= 1)
-            for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_mode);
+            for (auto& kvp : by_n) gram_solve(D, sol, kvp.second, opt.eig_mode, opt.roc_trd);
         else
-            gram_solve(D, sol, all, opt.eig_mode);
+            gram_solve(D, sol, all, opt.eig_mode, opt.roc_trd);
         // the isometric route's values: only min(rows, cols) are singular values
         for (int p = 0; p < np; ++p)
             if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
@@ -739,14 +794,14 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             if (X[p].iso) {
                 if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
                     opt.tail * X[p].s[0] >= s_last) {
-                    iso_refine(D, sol, X[p], opt.tail, opt.eig_mode);
+                    iso_refine(D, sol, X[p], opt.tail, opt.eig_mode, opt.roc_trd);
                     redo = true;
                 }
             } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
                 // U = M V / s would lose orthonormality at these values: redo the sector
                 // on its isometric (larger) side
                 gram_prepare(D, X[p], true, !left2right);
-                gram_solve(D, sol, {&X[p]}, opt.eig_mode);
+                gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
                 X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
                 redo = true;
             }

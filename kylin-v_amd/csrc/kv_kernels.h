@@ -79,4 +79,16 @@ void launch_sum_parts(hipStream_t st, const double2* part, int np, int64_t n, do
 // Z[j n + i] = C[(c0 + j) ldc + i]  (i < n, j < k): real columns to complex.
 void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, int n, double2* Z);
 
+// Batched Householder tridiagonalisation (lower, LAPACK zhetrd storage) of column-major
+// n x n Hermitian matrices at stride n * n: panel width kTrdPanel, workspace VW (n x 2 nb per
+// matrix), y (n), q (2 nb), part (trd_column_blocks(n, 0)), count (zeroed once).
+constexpr int kTrdPanel = 32;
+void launch_herm_fill(hipStream_t st, double2* A, int n, int batch);
+int trd_column_blocks(int n, int j);
+void launch_trd_column(hipStream_t st, double2* A, double2* VW, const double2* y, const double2* q, double* D,
+                       double* E, double2* tau, double2* part, unsigned* count, int n, int j, int i, int nb,
+                       bool prev, bool col, int batch);
+void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, double2* y, double2* q, int n, int s,
+                       int i, int nb, int batch);
+
 }  // namespace kv

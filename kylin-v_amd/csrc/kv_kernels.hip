@@ -1120,4 +1120,367 @@ void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, i
     KV_HIP(hipGetLastError());
 }
 
+
+// ---- Householder tridiagonalisation (lower triangle, LAPACK storage) ----------------
+// Batched reduction Q^H A Q = T of column-major n x n Hermitian matrices (one per grid.y),
+// blocked by panels of kTrdPanel columns: within a panel column j is brought up to date by
+// the panel's earlier reflectors (V) and their companions (W), reflected (larfg), and its
+// companion W(:, i) = tau (A v - V (W^H v) - W (V^H v)) + alpha v formed from a matvec over
+// the trailing matrix as it stood at the panel start; after the panel the trailing matrix
+// takes the rank-2 update A -= V W^H + W V^H on both triangles.  A is kept fully Hermitian
+// (both triangles) so the matvec is a set of independent column dot products.  On exit
+// D = diag(T), E = subdiag(T), tau and the reflector tails below the subdiagonal of A follow
+// LAPACK zhetrd('L').  VW holds V in columns [0, nb) and W in [nb, 2 nb) (column-major,
+// n rows); q holds W^H v in [0, nb) and V^H v in [nb, 2 nb).
+
+// Mirror the lower triangle of each column-major n x n matrix into its upper triangle
+// (conjugated) and drop the imaginary part of the diagonal.  32 x 32 tiles, grid (T, T, batch).
+__global__ __launch_bounds__(256) void herm_fill_kernel(double2* __restrict__ A, int n, int64_t stride) {
+    const int ti = blockIdx.x, tj = blockIdx.y;  // source tile rows ti, columns tj (lower: ti >= tj)
+    if (ti < tj) return;
+    A += (int64_t)blockIdx.z * stride;
+    __shared__ double2 t[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int c = ty; c < 32; c += 8) {
+        const int gr = ti * 32 + tx, gc = tj * 32 + c;
+        if (gr < n && gc < n) t[c][tx] = A[(int64_t)gc * n + gr];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int dc = ti * 32 + r, dr = tj * 32 + tx;  // destination (dr, dc) = conj(source (dc, dr))
+        if (dc < n && dr < n) {
+            const double2 v = t[tx][r];
+            if (dc > dr)
+                A[(int64_t)dc * n + dr] = make_double2(v.x, -v.y);
+            else if (dc == dr)
+                A[(int64_t)dc * n + dr] = make_double2(v.x, 0.0);
+        }
+    }
+}
+
+constexpr int kTrdRows = 64;  // rows per block of trd_column_kernel (x 4 slices of the panel index)
+constexpr int kTrdReg = 8;    // rows per thread whose loads are issued together (256 threads: 2048 rows)
+
+__device__ inline double2 cmul(double2 a, double2 b) {
+    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ inline double2 cmulc(double2 a, double2 b) {  // a * conj(b)
+    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ inline double2 wave_sum2(double2 v) {  // fixed butterfly over the 64 lanes
+    for (int off = 32; off > 0; off >>= 1) {
+        v.x += __shfl_xor(v.x, off, 64);
+        v.y += __shfl_xor(v.y, off, 64);
+    }
+    return v;
+}
+
+// One column step of the panel, rows [j, n), grid (ceil((n - j) / 64), batch).
+// prev: finish the companion of column j - 1 (local index i - 1): w' = tau_{j-1} (y - V q1 - W q2)
+//   over the panel's columns before i - 1, alpha = -tau_{j-1} (w'^H v) / 2, W(:, i-1) = w' + alpha v.
+// col: bring column j up to date with the panel's columns before i and reflect it (j < n - 1);
+//   with v(j) = 1 the contribution of column i - 1 is -v conj(w'(j)) - w' - 2 Re(alpha) v.
+// Each block writes its share of w'^H v; the last block to finish (ticket on count[b], which
+// it resets) sums the shares in block order and does the O(n) tail: W, the final column, its
+// norm, larfg and the reflector, with each thread's rows loaded together.
+__global__ __launch_bounds__(256) void trd_column_kernel(double2* __restrict__ A, double2* __restrict__ VW,
+                                                         const double2* __restrict__ y,
+                                                         const double2* __restrict__ q, double* __restrict__ Dg,
+                                                         double* __restrict__ Eg, double2* __restrict__ tau,
+                                                         double2* __restrict__ part, unsigned* __restrict__ count,
+                                                         int n, int j, int i, int nb, int prev, int col) {
+    const int b = blockIdx.y;
+    A += (int64_t)b * n * n;
+    VW += (int64_t)b * n * 2 * nb;
+    y += (int64_t)b * n;
+    q += (int64_t)b * 2 * nb;
+    Dg += (int64_t)b * n;
+    Eg += (int64_t)b * n;
+    tau += (int64_t)b * n;
+    part += (int64_t)b * gridDim.x;
+    const double2* V = VW;
+    double2* W = VW + (int64_t)nb * n;
+    const double2 zero2 = make_double2(0.0, 0.0);
+    const int tx = threadIdx.x & 63, ty = threadIdx.x >> 6;
+    const int K = prev ? i - 1 : 0;  // panel columns with final companions
+    __shared__ double2 vj[kTrdPanel], wj[kTrdPanel], q1[kTrdPanel], q2[kTrdPanel];
+    __shared__ double2 rw[4][kTrdRows], rc[4][kTrdRows], red[kTrdRows];
+    __shared__ double2 wpj_s, alpha_s, alf_s, scale_s;
+    __shared__ double ssq[4];
+    __shared__ int last;
+    const double2 taup = prev ? tau[j - 1] : zero2;
+    if (threadIdx.x < K) {
+        vj[threadIdx.x] = V[(int64_t)threadIdx.x * n + j];
+        wj[threadIdx.x] = W[(int64_t)threadIdx.x * n + j];
+        q1[threadIdx.x] = q[threadIdx.x];
+        q2[threadIdx.x] = q[nb + threadIdx.x];
+    }
+    const int r = j + blockIdx.x * kTrdRows + tx;
+    double2 yr = zero2, ar = zero2, vr = zero2;
+    if (ty == 0 && r < n) {
+        if (prev) {
+            yr = y[r];
+            vr = V[(int64_t)(i - 1) * n + r];
+        }
+        if (col) ar = A[(int64_t)j * n + r];
+    }
+    __syncthreads();
+    if (prev && threadIdx.x < 64) {  // w'(j), needed by every row of column j: one panel column per lane
+        double2 t = zero2;
+        if (threadIdx.x < K) {
+            const double2 a = cmul(vj[threadIdx.x], q1[threadIdx.x]), c = cmul(wj[threadIdx.x], q2[threadIdx.x]);
+            t = make_double2(a.x + c.x, a.y + c.y);
+        }
+        t = wave_sum2(t);
+        if (threadIdx.x == 0) {
+            const double2 yj = y[j];
+            wpj_s = cmul(taup, make_double2(yj.x - t.x, yj.y - t.y));
+        }
+    }
+    double2 sw = zero2, sc = zero2;
+    if (r < n) {
+        for (int k = ty; k < K; k += 4) {
+            const double2 vk = V[(int64_t)k * n + r], wk = W[(int64_t)k * n + r];
+            const double2 a = cmul(vk, q1[k]), c = cmul(wk, q2[k]);
+            sw.x += a.x + c.x;
+            sw.y += a.y + c.y;
+            const double2 e = cmulc(vk, wj[k]), f = cmulc(wk, vj[k]);
+            sc.x += e.x + f.x;
+            sc.y += e.y + f.y;
+        }
+    }
+    rw[ty][tx] = sw;
+    rc[ty][tx] = sc;
+    __syncthreads();
+    if (ty == 0) {
+        double2 d = zero2;
+        if (r < n) {
+            for (int t = 1; t < 4; ++t) {
+                sw.x += rw[t][tx].x;
+                sw.y += rw[t][tx].y;
+                sc.x += rc[t][tx].x;
+                sc.y += rc[t][tx].y;
+            }
+            double2 wp = zero2;
+            if (prev) {
+                wp = cmul(taup, make_double2(yr.x - sw.x, yr.y - sw.y));
+                W[(int64_t)(i - 1) * n + r] = wp;
+                d = cmulc(vr, wp);  // conj(wp) v
+            }
+            if (col) {
+                double2 c0 = make_double2(ar.x - sc.x, ar.y - sc.y);
+                if (prev) {
+                    const double2 t2 = cmulc(vr, wpj_s);
+                    c0.x -= t2.x + wp.x;
+                    c0.y -= t2.y + wp.y;
+                }
+                A[(int64_t)j * n + r] = c0;
+            }
+        }
+        d = wave_sum2(d);
+        if (tx == 0) {
+            part[blockIdx.x] = d;
+            __threadfence();
+            last = atomicAdd(&count[b], 1u) == gridDim.x - 1;
+        }
+    }
+    __syncthreads();
+    if (!last) return;
+    __threadfence();
+    // ---- tail, one block: rows [j, n) ----
+    if (threadIdx.x < 64) {
+        double2 d = zero2;
+        if (prev)
+            for (unsigned t = threadIdx.x; t < gridDim.x; t += 64) {
+                d.x += part[t].x;
+                d.y += part[t].y;
+            }
+        d = wave_sum2(d);
+        if (threadIdx.x == 0) {
+            alpha_s = prev ? cmul(make_double2(-0.5 * taup.x, -0.5 * taup.y), d) : zero2;
+            alf_s = zero2;
+            count[b] = 0u;
+        }
+    }
+    __syncthreads();
+    const double2 alpha = alpha_s;
+    const double g2 = 2.0 * alpha.x;
+    const int m = n - j;
+    const bool spill = m > 256 * kTrdReg;  // rows beyond the registers go back through A
+    double2 cv[kTrdReg];
+    double s2 = 0.0;
+    for (int base = 0; base < m; base += 256 * kTrdReg) {
+        double2 vv[kTrdReg], ww[kTrdReg], cc[kTrdReg];
+#pragma unroll
+        for (int u = 0; u < kTrdReg; ++u) {
+            const int rr = j + base + u * 256 + threadIdx.x;
+            vv[u] = zero2;
+            ww[u] = zero2;
+            cc[u] = zero2;
+            if (rr < n) {
+                if (prev) {
+                    vv[u] = V[(int64_t)(i - 1) * n + rr];
+                    ww[u] = W[(int64_t)(i - 1) * n + rr];
+                }
+                if (col) cc[u] = A[(int64_t)j * n + rr];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kTrdReg; ++u) {
+            const int rr = j + base + u * 256 + threadIdx.x;
+            if (rr >= n) continue;
+            if (prev) {
+                const double2 av = cmul(alpha, vv[u]);
+                W[(int64_t)(i - 1) * n + rr] = make_double2(ww[u].x + av.x, ww[u].y + av.y);
+            }
+            if (col) {
+                const double2 c = make_double2(cc[u].x - g2 * vv[u].x, cc[u].y - g2 * vv[u].y);
+                if (base == 0) cv[u] = c;
+                if (spill) A[(int64_t)j * n + rr] = c;
+                if (rr == j) Dg[j] = c.x;
+                if (rr == j + 1) alf_s = c;
+                if (rr >= j + 2) s2 += c.x * c.x + c.y * c.y;
+            }
+        }
+    }
+    if (!col || j >= n - 1) return;
+    {
+        double2 t = wave_sum2(make_double2(s2, 0.0));
+        if (tx == 0) ssq[ty] = t.x;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {  // larfg: H^H [alpha; x] = [beta; 0], H = I - tau v v^H, v = [1; x / (alpha - beta)]
+        const double2 a = alf_s;
+        const double xn2 = ((ssq[0] + ssq[1]) + ssq[2]) + ssq[3];
+        double beta, tr = 0.0, ti = 0.0;
+        double2 z = make_double2(1.0, 0.0);
+        if (xn2 == 0.0 && a.y == 0.0) {
+            beta = a.x;
+        } else {
+            beta = -copysign(sqrt(a.x * a.x + a.y * a.y + xn2), a.x);
+            tr = (beta - a.x) / beta;
+            ti = -a.y / beta;
+            const double zr = a.x - beta, zi = a.y, z2 = zr * zr + zi * zi;
+            z = make_double2(zr / z2, -zi / z2);
+        }
+        Eg[j] = beta;
+        tau[j] = make_double2(tr, ti);
+        scale_s = z;
+        A[(int64_t)j * n + j + 1] = make_double2(beta, 0.0);
+        VW[(int64_t)i * n + j + 1] = make_double2(1.0, 0.0);
+    }
+    __syncthreads();
+    const double2 scl = scale_s;
+    for (int base = 0; base < m; base += 256 * kTrdReg) {
+#pragma unroll
+        for (int u = 0; u < kTrdReg; ++u) {
+            const int rr = j + base + u * 256 + threadIdx.x;
+            if (rr < j + 2 || rr >= n) continue;
+            const double2 v = cmul(base == 0 ? cv[u] : A[(int64_t)j * n + rr], scl);
+            A[(int64_t)j * n + rr] = v;
+            VW[(int64_t)i * n + rr] = v;
+        }
+    }
+}
+
+// y[c] = sum_{r >= s} conj(A(r, c)) v(r) for c in [s, n) (= (A v)(c), A Hermitian), then
+// q[k] = W(:, k)^H v and q[nb + k] = V(:, k)^H v over rows [s, n) for k < i, with v = V(:, i).
+// Two outputs per block of 256 threads, the rows spread over the block with each thread's
+// loads issued together; grid (ceil((n - s + 2 i) / 2), batch).
+__global__ __launch_bounds__(256) void trd_matvec_kernel(const double2* __restrict__ A,
+                                                         const double2* __restrict__ VW, double2* __restrict__ y,
+                                                         double2* __restrict__ q, int n, int s, int i, int nb) {
+    const int b = blockIdx.y;
+    A += (int64_t)b * n * n;
+    VW += (int64_t)b * n * 2 * nb;
+    y += (int64_t)b * n;
+    q += (int64_t)b * 2 * nb;
+    const int m = n - s, ntask = m + 2 * i;
+    const int t0 = blockIdx.x * 2;
+    const bool two = t0 + 1 < ntask;
+    auto colp = [&](int t) -> const double2* {
+        if (t < m) return A + (int64_t)(s + t) * n;
+        t -= m;
+        return t < i ? VW + (int64_t)(nb + t) * n : VW + (int64_t)(t - i) * n;  // W(:, t) or V(:, t - i)
+    };
+    const double2* c0 = colp(t0);
+    const double2* c1 = two ? colp(t0 + 1) : c0;
+    const double2* v = VW + (int64_t)i * n;
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    for (int base = s; base < n; base += 256 * kTrdReg) {
+        double2 vv[kTrdReg], x0[kTrdReg], x1[kTrdReg];
+#pragma unroll
+        for (int u = 0; u < kTrdReg; ++u) {
+            const int r = base + u * 256 + threadIdx.x;
+            vv[u] = make_double2(0.0, 0.0);
+            x0[u] = vv[u];
+            x1[u] = vv[u];
+            if (r < n) {
+                vv[u] = v[r];
+                x0[u] = c0[r];
+                x1[u] = c1[r];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kTrdReg; ++u) {
+            const double2 e = cmulc(vv[u], x0[u]), f = cmulc(vv[u], x1[u]);  // conj(x) v
+            a0.x += e.x;
+            a0.y += e.y;
+            a1.x += f.x;
+            a1.y += f.y;
+        }
+    }
+    a0 = wave_sum2(a0);
+    a1 = wave_sum2(a1);
+    __shared__ double2 red[2][4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    if (lane == 0) {
+        red[0][wave] = a0;
+        red[1][wave] = a1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 && (threadIdx.x == 0 || two)) {
+        const int t = t0 + threadIdx.x;
+        double2 acc = red[threadIdx.x][0];
+        for (int w = 1; w < 4; ++w) {
+            acc.x += red[threadIdx.x][w].x;
+            acc.y += red[threadIdx.x][w].y;
+        }
+        if (t < m)
+            y[s + t] = acc;
+        else if (t - m < i)
+            q[t - m] = acc;
+        else
+            q[nb + t - m - i] = acc;
+    }
+}
+
+void launch_herm_fill(hipStream_t st, double2* A, int n, int batch) {
+    if (n <= 0 || batch <= 0) return;
+    const int T = (n + 31) / 32;
+    hipLaunchKernelGGL(herm_fill_kernel, dim3(T, T, batch), dim3(256), 0, st, A, n, (int64_t)n * n);
+    KV_HIP(hipGetLastError());
+}
+
+int trd_column_blocks(int n, int j) { return (n - j + kTrdRows - 1) / kTrdRows; }
+
+void launch_trd_column(hipStream_t st, double2* A, double2* VW, const double2* y, const double2* q, double* D,
+                       double* E, double2* tau, double2* part, unsigned* count, int n, int j, int i, int nb,
+                       bool prev, bool col, int batch) {
+    KV_REQUIRE(nb <= kTrdPanel && i <= nb && j >= 0 && j < n && (prev ? i >= 1 : i == 0) && j - i >= 0, ERR_INTERNAL,
+               "trd_column: bad step");
+    hipLaunchKernelGGL(trd_column_kernel, dim3(trd_column_blocks(n, j), batch), dim3(256), 0, st, A, VW, y, q, D,
+                       E, tau, part, count, n, j, i, nb, prev ? 1 : 0, col ? 1 : 0);
+    KV_HIP(hipGetLastError());
+}
+
+void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, double2* y, double2* q, int n, int s,
+                       int i, int nb, int batch) {
+    KV_REQUIRE(s >= 1 && s < n && i >= 0 && i < nb && nb <= kTrdPanel, ERR_INTERNAL, "trd_matvec: bad step");
+    const int ntask = n - s + 2 * i;
+    hipLaunchKernelGGL(trd_matvec_kernel, dim3((ntask + 1) / 2, batch), dim3(256), 0, st, A, VW, y, q, n, s, i,
+                       nb);
+    KV_HIP(hipGetLastError());
+}
+
 }  // namespace kv

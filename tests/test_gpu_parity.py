@@ -420,14 +420,20 @@ def test_mps_mpo_files_round_trip(kv, tmp_path):
     assert e.value.code == -5
 
 
-@pytest.mark.parametrize("eig", ["split", "heevd"])
+def _set_eig(monkeypatch, eig):
+    """split: own tridiagonalisation; split-roc: rocSOLVER zhetrd; heevd: zheevd."""
+    monkeypatch.setenv("KV_EIG", "split" if eig.startswith("split") else eig)
+    monkeypatch.setenv("KV_TRD", "rocsolver" if eig == "split-roc" else "own")
+
+
+@pytest.mark.parametrize("eig", ["split", "split-roc", "heevd"])
 def test_svd_gram_route_matches_oracle(kv, eig, monkeypatch):
     """Sectors whose smaller side exceeds 512 take the Gram route, with the split Hermitian
-    eigensolver (zhetrd, dstedc, back-transformation of the kept vectors) or zheevd: the
-    kept singular values, the truncated product and the isometry match the oracle's dense
-    SVD."""
+    eigensolver (tridiagonalisation, dstedc, back-transformation of the kept vectors) or
+    zheevd: the kept singular values, the truncated product and the isometry match the
+    oracle's dense SVD."""
     print(f"case svd gram eig={eig}", flush=True)
-    monkeypatch.setenv("KV_EIG", eig)
+    _set_eig(monkeypatch, eig)
     from kylin_v_amd import model as KM
     sites, terms, init, ex = KM.chain(6, [1.0], 1.0, [16], 1.0, "merged")
     blocks = KM.synthetic_state_blocks(sites, 64, seed=3, decay=0.3)
@@ -475,7 +481,7 @@ def _spectrum_tensor(rng, svals_by_q):
     return t
 
 
-@pytest.mark.parametrize("eig", ["split", "heevd"])
+@pytest.mark.parametrize("eig", ["split", "split-roc", "heevd"])
 @pytest.mark.parametrize("l2r", [True, False])
 def test_svd_isometric_route_tail_refinement(kv, l2r, eig, monkeypatch):
     """Sectors of 100 and 120 singular values spanning 1 .. 1e-14 with none in
@@ -484,7 +490,7 @@ def test_svd_isometric_route_tail_refinement(kv, l2r, eig, monkeypatch):
     truncated product and an isometry orthonormal to machine precision; a maxdim-limited
     cut must match too."""
     print(f"case svd iso l2r={l2r} eig={eig}", flush=True)
-    monkeypatch.setenv("KV_EIG", eig)
+    _set_eig(monkeypatch, eig)
     rng = np.random.default_rng(12)
 
     def spec(n, c):
