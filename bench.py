@@ -22,7 +22,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "TDVP steps/s + H_eff TFLOP/s vs FP64 GPU path at max|ΔC(t)|≤1e-6"
-PEAK = {"fp64": 78.6, "fp32": 157.3}  # MI355X dense MFMA TFLOP/s (AMD published)
+PEAK = {"fp64": 78.6, "fp32": 157.3,  # MI355X dense MFMA TFLOP/s (AMD published)
+        "hbm_gbs": 8000.0}  # MI355X HBM3E GB/s
 # HBM bytes per launch of zgemm_grouped_kernel<float> in this workload's mixed path, from
 # two rocprofv3 --pmc passes over `bench.py --steps 1 --no-cpu --physical-steps 0`
 # (FETCH_SIZE, WRITE_SIZE; mean over 1096 dispatches), each scaled by what the counter
@@ -144,6 +145,9 @@ def summarize(r, precision, steps):
                 heff_gemm_gflop_per_launch=st["heff_gemm_flop"] / max(1, st["heff_gemm_launches"]) / 1e9,
                 heff_gemm_gb_per_launch=st["heff_gemm_bytes"] / max(1, st["heff_gemm_launches"]) / 1e9,
                 fp32_gemm=gemm32,
+                svd_tridiag=dict(gpu_s=st["trd_gpu_s"], gb=st["trd_bytes"] / 1e9,
+                                 columns=int(st["trd_columns"]),
+                                 gb_per_s=st["trd_bytes"] / st["trd_gpu_s"] / 1e9 if st["trd_gpu_s"] > 0 else 0.0),
                 module_s={k: round(st[k], 3) for k in ("Evl-2", "Evl-1", "SVD", "Update")},
                 matvecs=int(st["matvecs"]), max_bond=max(r["bonds"]))
 
@@ -177,6 +181,7 @@ def main():
     cpu = None if args.no_cpu else cpu_baseline(kv, ch, psi0)
     g32 = s32["fp32_gemm"]
     ach = g32["tflops"]
+    trd = s32["svd_tridiag"]
     out = {
         "metric": METRIC,
         "value": s32["steps_per_s"],
@@ -211,7 +216,12 @@ def main():
                      "flop_per_launch": g32["gflop_per_launch"] * 1e9,
                      "ms_per_launch": g32["ms_per_launch"], "launches": g32["launches"],
                      "fp64_kernel": {"achieved": s64["heff_gemm_tflops"], "peak": PEAK["fp64"],
-                                     "frac": s64["heff_gemm_tflops"] / PEAK["fp64"]}},
+                                     "frac": s64["heff_gemm_tflops"] / PEAK["fp64"]},
+                     "svd_tridiag": {"kernel": "trd_column + trd_matvec + panel zgemm (bond SVD, FP64)",
+                                     "bound": "hbm", "achieved": trd["gb_per_s"], "peak": PEAK["hbm_gbs"],
+                                     "unit": "GB/s", "frac": trd["gb_per_s"] / PEAK["hbm_gbs"],
+                                     "gpu_s_per_step": trd["gpu_s"] / args.steps,
+                                     "columns": trd["columns"]}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -210,7 +210,11 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
  *   17 heff_gemm_bytes (algorithmic: each GEMM's operands and result counted once)
  *   18 heff_gemm32_gpu_s 19 heff_gemm32_flop 20 heff_gemm32_launches 21 heff_gemm32_bytes
  *      (the FP32-operand GEMM launches alone; the mixed path's first matvec per local
- *      step and every FP64-path launch are FP64) */
+ *      step and every FP64-path launch are FP64)
+ *   22 trd_gpu_s 23 trd_bytes 24 trd_columns: the bond SVD's Householder
+ *      tridiagonalisations on the engine's own kernels (HIP-event time; algorithmic bytes =
+ *      the Hermitian trailing triangle read once per column, 16 B per element; columns
+ *      summed over the matrices of each batched call) */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

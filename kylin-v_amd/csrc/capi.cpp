@@ -480,15 +480,17 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
     const kv::TdvpTimers& t = s->s->timers();
     const kv::KernelStats& h = s->s->heff_kernels();
     const kv::KernelStats& e = s->s->env_kernels();
-    const double v[22] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
+    const kv::TrdStats& r = s->s->trd_stats();
+    const double v[25] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
                           (h.gemm_ms + h.wfold_ms) * 1e-3, h.gemm_flop + h.wfold_flop,
                           (e.gemm_ms + e.wfold_ms) * 1e-3, e.gemm_flop + e.wfold_flop,
                           (double)s->s->lanczos_stats().matvecs,
                           h.gemm_ms * 1e-3, h.gemm_flop, (double)h.gemm_launches,
                           h.wfold_ms * 1e-3, h.wfold_flop, (double)h.wfold_launches,
                           h.gemm_bytes, h.gemm32_ms * 1e-3, h.gemm32_flop,
-                          (double)h.gemm32_launches, h.gemm32_bytes};
-    for (int i = 0; i < n && i < 22; ++i) out[i] = v[i];
+                          (double)h.gemm32_launches, h.gemm32_bytes,
+                          r.ms * 1e-3, r.bytes, (double)r.columns};
+    for (int i = 0; i < n && i < 25; ++i) out[i] = v[i];
     return KV_OK;
 }
 

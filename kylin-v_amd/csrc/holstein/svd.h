@@ -367,6 +367,8 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                                   hipMemcpyDeviceToDevice, D.stream));
     } else {
         taub = DevBuf(D.pool.get(), (size_t)N * nb * sizeof(double2));
+        EventTimer trd_ev;
+        bool trd_timed = false;
         if (roc_trd) {
             auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
             const rocblas_status st = rocsolver_zhetrd_strided_batched(
@@ -374,7 +376,10 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                 E.as<double>(), N, tp, N, nb);
             KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
         } else {
+            trd_ev.start(D.stream);
             hetrd_lower(D, sol, Gb.as<double2>(), N, nb, W.as<double>(), E.as<double>(), taub.as<double2>());
+            trd_ev.stop(D.stream);
+            trd_timed = true;
         }
         // the tridiagonal forms, kept for a redo (dstedc overwrites both arrays)
         DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
@@ -389,6 +394,11 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
         read_info();
+        if (trd_timed) {
+            D.trd.ms += trd_ev.ms();
+            for (int64_t m = 1; m < N; ++m) D.trd.bytes += 16.0 * (double)(m * (m + 1) / 2) * nb;
+            D.trd.columns += (long)N * nb;
+        }
         for (int b = 0; b < nb; ++b) {
             const int n = xs[b]->k;
             const int64_t nn = (int64_t)n * n;

@@ -232,7 +232,9 @@ class TDVP {
         env_flop_ = 0;
         heff_k_ = KernelStats();
         env_k_ = KernelStats();
+        dev_->trd = TrdStats();
     }
+    const TrdStats& trd_stats() const { return dev_->trd; }
     bool timing = true;
 
   private:

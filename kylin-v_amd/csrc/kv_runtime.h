@@ -159,12 +159,20 @@ struct KernelStats {
     long gemm32_launches = 0;
 };
 
+// The Householder tridiagonalisations of the bond SVD (hetrd_lower): HIP-event time, the
+// Hermitian trailing triangle each column's matvec must read (16 B per element), columns.
+struct TrdStats {
+    double ms = 0, bytes = 0;
+    long columns = 0;
+};
+
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
     std::unique_ptr<Pool> pool;
     bool timing = false;  // record HIP events around contraction kernels
     KernelStats stats;
+    TrdStats trd;
     // GEMM tile autotuner: (precision, opA, opB, shape class) -> tile config
     bool autotune = true;
     std::map<uint64_t, int> gemm_tuned;
