@@ -166,8 +166,11 @@ int kv_krylov_step(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* 
  * global descending sort, drop entry i (never 0) while i >= maxdim or s_i <= cutoff;
  * S absorbed into V^dag if left_to_right else into U.  nkept receives the kept bond
  * dimension.  Sectors whose smaller side is at least 64 are solved through their Gram
- * matrix on the isometric side (rocSOLVER Hermitian eigensolvers, a second pass over the
- * tail below 1e-5 of the largest value); smaller ones by rocSOLVER's Jacobi SVD. */
+ * matrix on the isometric side (Hermitian eigen-decomposition: from side 1024, or for
+ * sectors of different sides, the engine's batched Householder tridiagonalisation,
+ * rocSOLVER's dstedc and the kept vectors carried back through the reflectors; below that
+ * rocSOLVER's zheevd; a second pass over the tail below 1e-5 of the largest value);
+ * smaller ones by rocSOLVER's Jacobi SVD. */
 int kv_svd_truncate(kv_ctx* ctx, const kv_tensor* A, int r1, double cutoff, int maxdim,
                     int left_to_right, kv_tensor** left, kv_tensor** right, int* nkept);
 
