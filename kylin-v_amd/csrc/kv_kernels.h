@@ -80,8 +80,10 @@ void launch_sum_parts(hipStream_t st, const double2* part, int np, int64_t n, do
 void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, int n, double2* Z);
 
 // Batched Householder tridiagonalisation (lower, LAPACK zhetrd storage) of column-major
-// n x n Hermitian matrices at stride n * n: panel width kTrdPanel, workspace VW (n x 2 nb per
-// matrix), y (n), q (2 nb), part (trd_column_blocks(n, 0)), count (zeroed once).
+// n x n Hermitian matrices at stride n * n, driven column by column from the host
+// (hetrd_lower): panel width kTrdPanel; workspace per matrix VW (n x 2 nb), y (n), q (2 nb),
+// part (trd_column_blocks(n, 0) complex) and count (one unsigned, zeroed before the first
+// step; every step leaves it zero).
 constexpr int kTrdPanel = 32;
 void launch_herm_fill(hipStream_t st, double2* A, int n, int batch);
 int trd_column_blocks(int n, int j);
