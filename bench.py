@@ -1,18 +1,24 @@
 """Benchmark: TDVP steps/s and H_eff TFLOP/s of the MI355X engine on the 32-site Holstein
-chain (16 phonon levels, bond dimension 256, dt = 0.02, two-site TDVP), mixed-precision
-(complex-FP32 contractions, FP64 Lanczos/SVD) against the same GPU's plain FP64 path.
+chain (16 phonon levels, bond dimension 256, dt = 0.02, two-site TDVP): the mixed-precision
+path (complex-FP32 H_eff contractions on MFMA; FP64 environments, Lanczos, exponential and
+bond SVD) against the same GPU's plain FP64 path, with the split-BF16 H_eff path beside them.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--physical-steps P]
 
-A step is one full two-site TDVP sweep pair (TDVP::two_site).  The start state is a
-seeded synthetic right-canonical MPS with every bond at D = 256 (the steady-state shape
-of the workload); the chain parameters are the workload's (g_i ~ U[0.8, 1.2] from
-numpy default_rng(2024)).  Inputs are resident in HBM before timing starts.
-Prints one JSON line.
+A step is one full two-site TDVP sweep pair (TDVP::two_site).  The timed start state is a
+seeded synthetic right-canonical MPS with every bond at D = 256 (the steady-state shape of
+the workload); the chain parameters are the workload's (g_i ~ U[0.8, 1.2] from numpy
+default_rng(2024)).  Inputs are resident in HBM before timing starts.  Outside the timed
+region every step's C(t) = <psi(0)|psi(t)> and bond energies E_{i,i+1} = <J(a+_i a_{i+1} +
+h.c.)> are measured; the mixed paths' errors against the FP64 path are reported, from the
+synthetic start and over P steps from the physical start (exciton on site 0, phonon
+vacuum), where the CPU leg also runs the oracle for the errors against the FP64 CPU
+algorithm.  Prints one JSON line.
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -22,28 +28,27 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "TDVP steps/s + H_eff TFLOP/s vs FP64 GPU path at max|ΔC(t)|≤1e-6"
-PEAK = {"fp64": 78.6, "fp32": 157.3,  # MI355X dense MFMA TFLOP/s (AMD published)
+GATE = 1e-6
+PEAK = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0 / 3,  # MI355X dense MFMA TFLOP/s (AMD)
         "hbm_gbs": 8000.0}  # MI355X HBM3E GB/s
-# HBM bytes per launch of zgemm_grouped_kernel<float> in this workload's mixed path, from
-# two rocprofv3 --pmc passes over `bench.py --steps 1 --no-cpu --physical-steps 0`
-# (FETCH_SIZE, WRITE_SIZE; mean over 1096 dispatches), each scaled by what the counter
-# reads for f64_to_f32_kernel moving a known 16+8 bytes per element (FETCH_SIZE x2.0,
-# WRITE_SIZE x1.0).  Summary and commands: profiles/chain32_merged_D256_gemm_f32_hbm_traffic.md
-GEMM_F32_TRAFFIC_BYTES = 0.414e9
 N, L, OMEGA, J, DT, D, CUTOFF = 32, 16, 0.5, 1.0, 0.02, 256, 1e-10
-
-
-def workload(kv, KM):
-    rng = np.random.default_rng(2024)
-    g = rng.uniform(0.8, 1.2, size=N)
-    ch = KM.HolsteinChain(N, [OMEGA], g, [L], J, "merged")
-    blocks = KM.synthetic_state_blocks(ch.sites, D, seed=7)
-    psi0 = [kv.Tensor.from_blocks((1, 1, -1), b) for b in blocks]
-    return ch, psi0
+PATHS = ("fp64", "fp32", "bf16x3")
+PROFILES = os.path.join(ROOT, "profiles")
 
 
 def log(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
+def couplings():
+    return np.random.default_rng(2024).uniform(0.8, 1.2, size=N)
+
+
+def workload(kv, KM):
+    ch = KM.HolsteinChain(N, [OMEGA], couplings(), [L], J, "merged")
+    blocks = KM.synthetic_state_blocks(ch.sites, D, seed=7)
+    psi0 = [kv.Tensor.from_blocks((1, 1, -1), b) for b in blocks]
+    return ch, psi0
 
 
 def run_path(kv, ch, psi0, precision, steps, warmup):
@@ -54,65 +59,63 @@ def run_path(kv, ch, psi0, precision, steps, warmup):
         eng.two_site()
         log(f"{precision}: warmup step {w + 1} done")
     eng.reset_stats()
-    cts, wall = [], 0.0
+    cts, energies, wall = [], [], 0.0
     for _ in range(steps):
         ch.ctx.sync()
         t0 = time.perf_counter()
         eng.two_site()
         ch.ctx.sync()
         wall += time.perf_counter() - t0
-        cts.append(kv.overlap_states(psi0, eng.mps))  # observable, outside the timed region
+        mps = eng.mps  # observables outside the timed region
+        cts.append(kv.overlap_states(psi0, mps))
+        energies.append(ch.bond_energies(mps))
         log(f"{precision}: timed step done, {wall:.1f} s so far")
     st = eng.stats()
     log(f"{precision}: modules " + ", ".join(f"{k}={v:.3g}" for k, v in st.items()))
-    return dict(wall=wall, ct=cts, stats=st, bonds=eng.bond_dims())
+    return dict(wall=wall, ct=np.array(cts), energies=np.array(energies), stats=st,
+                bonds=eng.bond_dims())
 
 
-def physical_gate(kv, ch, steps):
-    """max|C_mixed(t) - C_fp64(t)| from the physical start (exciton on site 0, phonon
-    vacuum), C(t) = <psi(0)|psi(t)>; not timed."""
-    cts = {}
-    for precision in ("fp64", "fp32"):
-        eng = kv.TDVP(ch.psi0, ch.mpo, cutoff=CUTOFF, maxdim=D, dt=DT, precision=precision)
-        cts[precision] = []
-        for k in range(steps):
-            eng.two_site()
-            cts[precision].append(kv.overlap_states(ch.psi0, eng.mps))
-        log(f"physical start {precision}: {steps} steps, max bond {max(eng.bond_dims())}")
-    c64, c32 = np.array(cts["fp64"]), np.array(cts["fp32"])
-    return float(np.max(np.abs(c32 - c64))), [[float(c.real), float(c.imag)] for c in c64]
+def physical_run(kv, ch, precision, steps):
+    """C(t) and bond energies of every step from the physical start (exciton on site 0,
+    phonon vacuum); not timed."""
+    eng = kv.TDVP(ch.psi0, ch.mpo, cutoff=CUTOFF, maxdim=D, dt=DT, precision=precision)
+    cts, energies = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        eng.two_site()
+        mps = eng.mps
+        cts.append(kv.overlap_states(ch.psi0, mps))
+        energies.append(ch.bond_energies(mps))
+    log(f"physical start {precision}: {steps} steps in {time.perf_counter() - t0:.1f} s, "
+        f"max bond {max(eng.bond_dims())}")
+    return dict(ct=np.array(cts), energies=np.array(energies), max_bond=max(eng.bond_dims()))
 
 
-def cpu_baseline(kv, ch, psi0):
-    """The oracle (numpy restatement of the reference algorithm) on one bond update of the
-    same workload: 2-site Lanczos + SVD + environment update + 1-site backward Lanczos at
-    the middle bond; a step is 2*(N-1) such updates."""
+def cpu_baseline(kv, ch, psi0, phys_steps):
+    """The oracle (numpy restatement of the reference algorithm, FP64 on the host):
+    (1) timed on one middle-bond update of the benchmark workload (2-site Lanczos + SVD +
+    environment update + 1-site backward Lanczos at bond 15-16; a step is 2 (N - 1) such
+    updates); (2) the FP64 CPU reference trajectory from the physical start for
+    `phys_steps` steps, C(t) and bond energies, against which the GPU paths' errors are
+    reported."""
+    from oracle import model as OM
     from oracle import tdvp as OT
-    from oracle.blocksparse import Tensor as OTensor
-    from oracle.blocksparse import product as oproduct
+    from kylin_v_amd import model as KM
 
     def down(t):
+        from oracle.blocksparse import Tensor as OTensor
         dirs, blocks = t.to_blocks()
         return OTensor(dirs, blocks)
 
-    A = list(psi0)
-    envL = kv.Tensor.from_blocks((1, -1, -1), {(0, 0, 0): np.ones((1, 1, 1))})
-    envR = kv.Tensor.from_blocks((-1, 1, 1), {(1, 0, 1): np.ones((1, 1, 1))})
     mid = N // 2 - 1
-    for i in range(mid):
-        cur, nex = kv.svd_impl(A[i], 2, -1.0, 10 ** 6, True)
-        A[i] = cur
-        A[i + 1] = kv.product(nex, A[i + 1], 1)
-        envL = kv.env_sweep(envL, A[i], ch.mpo[i], True)
-    envR2 = envR
-    for i in range(N - 1, mid + 1, -1):
-        envR2 = kv.env_sweep(envR2, A[i], ch.mpo[i], False)
-    envR1 = kv.env_sweep(envR2, A[mid + 1], ch.mpo[mid + 1], False)
-    L0, R2, R1 = down(envL), down(envR2), down(envR1)
+    envL, a, b, envR2, envR1 = KM.centre_bond(psi0, ch.mpo, mid)
+    L0, R2 = down(envL), down(envR2)
     W1, W2 = down(ch.mpo[mid]), down(ch.mpo[mid + 1])
-    a, b = down(A[mid]), down(A[mid + 1])
+    a, b = down(a), down(b)
     threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     log("cpu baseline: environments ready, timing the oracle bond update")
+    from oracle.blocksparse import product as oproduct
     t0 = time.perf_counter()
     x = oproduct(a, b, 1)
     x, _ = OT.local_krylov(L0, x, W1, W2, R2, DT, 1e-8, 15, False, 2)
@@ -120,46 +123,121 @@ def cpu_baseline(kv, ch, psi0):
     L1 = OT.env_sweep(L0, cur, W1, True)
     nex, _ = OT.local_krylov(L1, nex, W2, None, R2, DT, 1e-8, 15, True, 1)
     t_bond = time.perf_counter() - t0
-    return dict(value=1.0 / (2 * (N - 1) * t_bond), unit="TDVP steps/s", cores=threads,
-                kind="port",
-                sample=(f"oracle (numpy/LAPACK, {threads} BLAS threads) one middle-bond update "
-                        f"of the same workload ({t_bond:.1f} s: 2-site Lanczos + SVD + env "
-                        f"update + 1-site Lanczos), scaled by the {2 * (N - 1)} bond updates of a step"))
+    out = dict(value=1.0 / (2 * (N - 1) * t_bond), unit="TDVP steps/s", cores=threads, kind="port",
+               sample=(f"oracle (numpy/LAPACK, {threads} BLAS threads) one middle-bond update "
+                       f"of the same workload ({t_bond:.1f} s: 2-site Lanczos + SVD + env "
+                       f"update + 1-site Lanczos), scaled by the {2 * (N - 1)} bond updates of a step"))
+    if phys_steps > 0:
+        lat = OM.holstein(N, [OMEGA], couplings().reshape(N, 1), [L], J, "merged")
+        mpo = OM.build_mpo(lat)
+        psi_0 = OM.product_state(lat)
+        ref = OT.TDVP(OM.product_state(lat), mpo, cutoff=CUTOFF, maxdim=D, dt=DT)
+        hop = [OM.hopping_mpo(lat, i) for i in range(N - 1)]
+        cts, energies = [], []
+        t0 = time.perf_counter()
+        for _ in range(phys_steps):
+            ref.two_site()
+            cts.append(OT.overlap_states(psi_0, ref.mps))
+            energies.append([OT.join(h, ref.mps, ref.mps).real for h in hop])
+        log(f"cpu baseline: oracle physical-start run, {phys_steps} steps in "
+            f"{time.perf_counter() - t0:.1f} s")
+        out["reference_run"] = dict(ct=np.array(cts), energies=np.array(energies))
+    return out
+
+
+def traffic_from_profiles(kernel_key):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summaries
+    (a line `bench-traffic <kernel_key> <bytes>`), or None."""
+    if not os.path.isdir(PROFILES):
+        return None, None
+    pat = re.compile(r"bench-traffic\s+(\S+)\s+([0-9.eE+-]+)")
+    for name in sorted(os.listdir(PROFILES)):
+        if not name.endswith((".md", ".txt")):
+            continue
+        with open(os.path.join(PROFILES, name)) as f:
+            for line in f:
+                m = pat.search(line)
+                if m and m.group(1) == kernel_key:
+                    return float(m.group(2)), f"profiles/{name}"
+    return None, None
+
+
+def kernel_table(st, steps):
+    """GPU seconds per step of each kernel class, from the engine's HIP-event counters."""
+    g64 = st["heff_gemm_gpu_s"] - st["heff_gemm32_gpu_s"]
+    return {
+        "heff_gemm_fp32_operands": st["heff_gemm32_gpu_s"] / steps,
+        "heff_gemm_fp64": g64 / steps,
+        "heff_mpo_fold": st["heff_wfold_gpu_s"] / steps,
+        "env_update": st["env_gpu_s"] / steps,
+        "svd_block_gemm_fp64": st["blk_gemm_gpu_s"] / steps,
+        "svd_block_cholesky_qr": st["blk_orth_gpu_s"] / steps,
+        "svd_block_rayleigh_ritz_eig": st["blk_eig_gpu_s"] / steps,
+        "svd_tridiagonalisation": st["trd_gpu_s"] / steps,
+    }
+
+
+def rooflines(st, precision):
+    """Roofline entries for the kernels that have one, each from algorithmic work per launch
+    over the HIP-event time per launch."""
+    out = {}
+    n32 = st["heff_gemm32_launches"]
+    if n32 > 0 and st["heff_gemm32_gpu_s"] > 0:
+        ach = st["heff_gemm32_flop"] / st["heff_gemm32_gpu_s"] / 1e12
+        peak = PEAK["bf16x3"] if precision == "bf16x3" else PEAK["fp32"]
+        out["heff_gemm_fp32_operands"] = {
+            "kernel": ("zgemm_bf16x3_kernel" if precision == "bf16x3" else "zgemm_grouped_kernel<float>")
+                      + " (H_eff stages)",
+            "bound": "mfma", "achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+            "flop_per_launch": st["heff_gemm32_flop"] / n32,
+            "algorithmic_bytes_per_launch": st["heff_gemm32_bytes"] / n32,
+            "ms_per_launch": 1e3 * st["heff_gemm32_gpu_s"] / n32, "launches": int(n32)}
+    nb = st["blk_gemm_launches"]
+    if nb > 0 and st["blk_gemm_gpu_s"] > 0:
+        ach = st["blk_gemm_flop"] / st["blk_gemm_gpu_s"] / 1e12
+        out["svd_block_gemm_fp64"] = {
+            "kernel": "zgemm_grouped_kernel<double> (bond SVD block route)", "bound": "mfma",
+            "achieved": ach, "peak": PEAK["fp64"], "unit": "TFLOP/s", "frac": ach / PEAK["fp64"],
+            "flop_per_launch": st["blk_gemm_flop"] / nb,
+            "algorithmic_bytes_per_launch": st["blk_gemm_bytes"] / nb,
+            "ms_per_launch": 1e3 * st["blk_gemm_gpu_s"] / nb, "launches": int(nb)}
+    if st["trd_gpu_s"] > 0:
+        ach = st["trd_bytes"] / st["trd_gpu_s"] / 1e9
+        out["svd_tridiagonalisation"] = {
+            "kernel": "trd_column + trd_matvec + panel zgemm (bond SVD full route)", "bound": "hbm",
+            "achieved": ach, "peak": PEAK["hbm_gbs"], "unit": "GB/s", "frac": ach / PEAK["hbm_gbs"],
+            "columns": int(st["trd_columns"])}
+    return out
 
 
 def summarize(r, precision, steps):
     st = r["stats"]
     heff_tf = st["heff_flop"] / st["heff_gpu_s"] / 1e12 if st["heff_gpu_s"] > 0 else 0.0
-    gemm_tf = st["heff_gemm_flop"] / st["heff_gemm_gpu_s"] / 1e12 if st["heff_gemm_gpu_s"] > 0 else 0.0
-    n32 = max(1, st["heff_gemm32_launches"])
-    gemm32 = dict(tflops=st["heff_gemm32_flop"] / st["heff_gemm32_gpu_s"] / 1e12
-                  if st["heff_gemm32_gpu_s"] > 0 else 0.0,
-                  launches=int(st["heff_gemm32_launches"]),
-                  ms_per_launch=1e3 * st["heff_gemm32_gpu_s"] / n32,
-                  gflop_per_launch=st["heff_gemm32_flop"] / n32 / 1e9,
-                  gb_per_launch=st["heff_gemm32_bytes"] / n32 / 1e9)
     return dict(steps_per_s=steps / r["wall"], ms_per_step=1e3 * r["wall"] / steps,
-                heff_tflops=heff_tf, heff_gemm_tflops=gemm_tf,
-                heff_gemm_launches=int(st["heff_gemm_launches"]),
-                heff_gemm_ms_per_launch=1e3 * st["heff_gemm_gpu_s"] / max(1, st["heff_gemm_launches"]),
-                heff_gemm_gflop_per_launch=st["heff_gemm_flop"] / max(1, st["heff_gemm_launches"]) / 1e9,
-                heff_gemm_gb_per_launch=st["heff_gemm_bytes"] / max(1, st["heff_gemm_launches"]) / 1e9,
-                fp32_gemm=gemm32,
-                svd_tridiag=dict(gpu_s=st["trd_gpu_s"], gb=st["trd_bytes"] / 1e9,
-                                 columns=int(st["trd_columns"]),
-                                 gb_per_s=st["trd_bytes"] / st["trd_gpu_s"] / 1e9 if st["trd_gpu_s"] > 0 else 0.0),
-                module_s={k: round(st[k], 3) for k in ("Evl-2", "Evl-1", "SVD", "Update")},
+                heff_tflops=heff_tf,
+                kernel_gpu_s_per_step={k: round(v, 4) for k, v in kernel_table(st, steps).items()},
+                rooflines=rooflines(st, precision),
+                module_s_per_step={k: round(st[k] / steps, 3) for k in ("Evl-2", "Evl-1", "SVD", "Update")},
+                svd_block=dict(sectors=int(st["blk_sectors"]), fallbacks=int(st["blk_fallbacks"]),
+                               eig_calls=int(st["blk_eig_calls"])),
                 matvecs=int(st["matvecs"]), max_bond=max(r["bonds"]))
+
+
+def max_abs(a, b):
+    return float(np.max(np.abs(np.asarray(a) - np.asarray(b))))
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=1)
-    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--physical-steps", type=int, default=6,
-                    help="steps of the physical-start run used for the mixed-vs-FP64 gate")
+    ap.add_argument("--physical-steps", type=int, default=100,
+                    help="steps of the physical-start runs used for the accuracy gate")
+    ap.add_argument("--oracle-steps", type=int, default=20,
+                    help="physical-start steps of the oracle (CPU leg) for errors against FP64 CPU")
+    ap.add_argument("--paths", default=",".join(PATHS))
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU: independent chains would shard across GPUs with no "
@@ -169,19 +247,54 @@ def main():
     kv = GE.load_package()
     from kylin_v_amd import model as KM
 
+    paths = [p for p in args.paths.split(",") if p]
+    assert "fp64" in paths and "fp32" in paths, "the headline needs the fp64 and fp32 paths"
     ch, psi0 = workload(kv, KM)
     log("workload ready")
-    r64 = run_path(kv, ch, psi0, "fp64", args.steps, args.warmup)
-    r32 = run_path(kv, ch, psi0, "fp32", args.steps, args.warmup)
-    s64 = summarize(r64, "fp64", args.steps)
-    s32 = summarize(r32, "fp32", args.steps)
-    dC = float(np.max(np.abs(np.array(r32["ct"]) - np.array(r64["ct"]))))
-    dC_phys, ct_phys = (physical_gate(kv, ch, args.physical_steps) if args.physical_steps > 0
-                        else (None, None))
-    cpu = None if args.no_cpu else cpu_baseline(kv, ch, psi0)
-    g32 = s32["fp32_gemm"]
-    ach = g32["tflops"]
-    trd = s32["svd_tridiag"]
+    runs = {p: run_path(kv, ch, psi0, p, args.steps, args.warmup) for p in paths}
+    summ = {p: summarize(runs[p], p, args.steps) for p in paths}
+    r64 = runs["fp64"]
+    for p in paths:
+        if p != "fp64":
+            summ[p]["max_dC_vs_fp64"] = max_abs(runs[p]["ct"], r64["ct"])
+            summ[p]["max_dE_bond_vs_fp64"] = max_abs(runs[p]["energies"], r64["energies"])
+            summ[p]["speedup_vs_fp64"] = summ[p]["steps_per_s"] / summ["fp64"]["steps_per_s"]
+            summ[p]["heff_speedup_vs_fp64"] = (summ[p]["heff_tflops"] / summ["fp64"]["heff_tflops"]
+                                               if summ["fp64"]["heff_tflops"] else None)
+    phys = {}
+    if args.physical_steps > 0:
+        phys = {p: physical_run(kv, ch, p, args.physical_steps) for p in paths}
+        for p in paths:
+            if p != "fp64":
+                summ[p]["max_dC_vs_fp64_physical_start"] = max_abs(phys[p]["ct"], phys["fp64"]["ct"])
+                summ[p]["max_dE_bond_vs_fp64_physical_start"] = max_abs(phys[p]["energies"],
+                                                                        phys["fp64"]["energies"])
+    cpu = None if args.no_cpu else cpu_baseline(kv, ch, psi0, min(args.oracle_steps, args.physical_steps))
+    if cpu is not None and "reference_run" in cpu:
+        ref = cpu.pop("reference_run")
+        k = len(ref["ct"])
+        for p in paths:
+            summ[p]["max_dC_vs_cpu_fp64_reference"] = max_abs(phys[p]["ct"][:k], ref["ct"])
+            summ[p]["max_dE_bond_vs_cpu_fp64_reference"] = max_abs(phys[p]["energies"][:k], ref["energies"])
+        cpu["reference_run_steps"] = k
+    gate = {p: dict(synthetic=summ[p]["max_dC_vs_fp64"],
+                    physical=summ[p].get("max_dC_vs_fp64_physical_start"))
+            for p in paths if p != "fp64"}
+    for p, g in gate.items():
+        g["passed"] = bool(g["synthetic"] <= GATE and (g["physical"] is None or g["physical"] <= GATE))
+    # the roofline of the kernel class with the most GPU time per step in the headline path
+    s32 = summ["fp32"]
+    kt = s32["kernel_gpu_s_per_step"]
+    with_roof = {k: v for k, v in kt.items() if k in s32["rooflines"]}
+    dominant = max(kt, key=kt.get)
+    primary_key = dominant if dominant in with_roof else max(with_roof, key=with_roof.get)
+    primary = dict(s32["rooflines"][primary_key])
+    primary["kernel_class"] = primary_key
+    primary["gpu_s_per_step"] = kt[primary_key]
+    primary["dominant_kernel_class"] = dominant
+    traffic, src = traffic_from_profiles(primary_key)
+    primary["traffic"] = traffic
+    primary["traffic_source"] = src
     out = {
         "metric": METRIC,
         "value": s32["steps_per_s"],
@@ -193,7 +306,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c64 (H_eff/env contractions) + f64 (Lanczos, SVD)",
+        "dtype": "c64 H_eff contractions (complex FP32 on MFMA, FP32 accumulation); f64 "
+                 "environments, Lanczos, Krylov exponential and bond SVD",
         "data": "synthetic",
         "config": {"workload": "Holstein chain N=32, 16 phonon levels, w=0.5, J=1, g_i~U[0.8,1.2] "
                                "(seed 2024), two-site TDVP dt=0.02, D=256, cutoff 1e-10, merged "
@@ -201,27 +315,14 @@ def main():
                                "right-canonical start (seed 7)",
                    "layout": "merged", "N": N, "L": L, "D": D, "dt": DT},
         "heff_tflops": s32["heff_tflops"],
-        "fp64_path": s64,
-        "mixed_path": s32,
-        "speedup_vs_fp64": s32["steps_per_s"] / s64["steps_per_s"],
-        "heff_speedup_vs_fp64": (s32["heff_tflops"] / s64["heff_tflops"]) if s64["heff_tflops"] else None,
-        "max_dC_vs_fp64": dC,
-        "max_dC_vs_fp64_physical_start": dC_phys,
+        "speedup_vs_fp64": s32["speedup_vs_fp64"],
+        "heff_speedup_vs_fp64": s32["heff_speedup_vs_fp64"],
+        "max_dC_vs_fp64": s32["max_dC_vs_fp64"],
+        "max_dE_bond_vs_fp64": s32["max_dE_bond_vs_fp64"],
+        "gate": gate,
         "physical_steps": args.physical_steps,
-        "C_t_fp64_physical_start": ct_phys,
-        "roofline": {"kernel": "zgemm_grouped_kernel<float> (H_eff matvec stages)", "bound": "mfma",
-                     "achieved": ach, "peak": PEAK["fp32"], "unit": "TFLOP/s",
-                     "frac": ach / PEAK["fp32"], "traffic": GEMM_F32_TRAFFIC_BYTES,
-                     "algorithmic_bytes_per_launch": g32["gb_per_launch"] * 1e9,
-                     "flop_per_launch": g32["gflop_per_launch"] * 1e9,
-                     "ms_per_launch": g32["ms_per_launch"], "launches": g32["launches"],
-                     "fp64_kernel": {"achieved": s64["heff_gemm_tflops"], "peak": PEAK["fp64"],
-                                     "frac": s64["heff_gemm_tflops"] / PEAK["fp64"]},
-                     "svd_tridiag": {"kernel": "trd_column + trd_matvec + panel zgemm (bond SVD, FP64)",
-                                     "bound": "hbm", "achieved": trd["gb_per_s"], "peak": PEAK["hbm_gbs"],
-                                     "unit": "GB/s", "frac": trd["gb_per_s"] / PEAK["hbm_gbs"],
-                                     "gpu_s_per_step": trd["gpu_s"] / args.steps,
-                                     "columns": trd["columns"]}},
+        "paths": summ,
+        "roofline": primary,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -165,12 +165,17 @@ int kv_krylov_step(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* 
 /* svd_impl<R1,R2>(A, cfg, LeftDir=-1) (holstein/svd.h:296-440): sector SVD in FP64,
  * global descending sort, drop entry i (never 0) while i >= maxdim or s_i <= cutoff;
  * S absorbed into V^dag if left_to_right else into U.  nkept receives the kept bond
- * dimension.  Sectors whose smaller side is at least 64 are solved through their Gram
- * matrix on the isometric side (Hermitian eigen-decomposition: from side 1024, or for
- * sectors of different sides, the engine's batched Householder tridiagonalisation,
- * rocSOLVER's dstedc and the kept vectors carried back through the reflectors; below that
- * rocSOLVER's zheevd; a second pass over the tail below 1e-5 of the largest value);
- * smaller ones by rocSOLVER's Jacobi SVD. */
+ * dimension.  Sectors whose smaller side is at least twice maxdim + 32 take a block
+ * subspace route: the isometric side's leading singular subspace from a seeded random
+ * block of maxdim + 32 columns with power steps, Cholesky QR between the products and
+ * Rayleigh-Ritz, each Ritz pair's residual checked against 1e-13 of the largest value;
+ * a cut the certified values cannot settle sends the sector to the full route.  Otherwise
+ * sectors whose smaller side is at least 64 are solved through their Gram matrix on the
+ * isometric side (Hermitian eigen-decomposition: from side 1024, or for sectors of
+ * different sides, the engine's batched Householder tridiagonalisation, rocSOLVER's dstedc
+ * and the kept vectors carried back through the reflectors; below that rocSOLVER's zheevd;
+ * a second pass over the tail below 1e-5 of the largest value); smaller ones by rocSOLVER's
+ * Jacobi SVD.  KV_SVD=full in the environment turns the block route off. */
 int kv_svd_truncate(kv_ctx* ctx, const kv_tensor* A, int r1, double cutoff, int maxdim,
                     int left_to_right, kv_tensor** left, kv_tensor** right, int* nkept);
 
@@ -217,7 +222,14 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
  *   22 trd_gpu_s 23 trd_bytes 24 trd_columns: the bond SVD's Householder
  *      tridiagonalisations on the engine's own kernels (HIP-event time; algorithmic bytes =
  *      the Hermitian trailing triangle read once per column, 16 B per element; columns
- *      summed over the matrices of each batched call) */
+ *      summed over the matrices of each batched call)
+ *   25 blk_gemm_gpu_s 26 blk_gemm_flop 27 blk_gemm_launches 28 blk_gemm_bytes: the FP64
+ *      GEMM launches of the bond SVD's block subspace route (HIP-event time; 8 m n k flop,
+ *      operands and result once each)
+ *   29 blk_orth_gpu_s 30 blk_orth_calls: its Cholesky QR kernels (factor + inverse)
+ *   31 blk_eig_gpu_s 32 blk_eig_calls: its Rayleigh-Ritz eigensolves
+ *   33 blk_sectors 34 blk_fallbacks: sectors solved by the route, and sectors it handed to
+ *      the full eigensolver */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

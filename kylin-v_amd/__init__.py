@@ -457,7 +457,9 @@ STAT_NAMES = ["Evl-2", "Evl-1", "Evl-0", "SVD", "Update", "Sweep", "heff_gpu_s",
               "env_gpu_s", "env_flop", "matvecs", "heff_gemm_gpu_s", "heff_gemm_flop",
               "heff_gemm_launches", "heff_wfold_gpu_s", "heff_wfold_flop", "heff_wfold_launches",
               "heff_gemm_bytes", "heff_gemm32_gpu_s", "heff_gemm32_flop", "heff_gemm32_launches",
-              "heff_gemm32_bytes", "trd_gpu_s", "trd_bytes", "trd_columns"]
+              "heff_gemm32_bytes", "trd_gpu_s", "trd_bytes", "trd_columns", "blk_gemm_gpu_s",
+              "blk_gemm_flop", "blk_gemm_launches", "blk_gemm_bytes", "blk_orth_gpu_s", "blk_orth_calls",
+              "blk_eig_gpu_s", "blk_eig_calls", "blk_sectors", "blk_fallbacks"]
 
 
 class TDVP:

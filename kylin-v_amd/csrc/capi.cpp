@@ -63,6 +63,17 @@ kv::Prec prec_of(int p) {
     return p == KV_FP64 ? kv::PREC_F64 : p == KV_FP32 ? kv::PREC_F32 : kv::PREC_BF16X3;
 }
 
+// Calls f(DotPair{b block offset, a block offset, size}) for every block of a whose qn and
+// shape b also has, in a's block order (overlap(a, b) = <b|a>, holstein/tensor.h:225-241).
+template <class F>
+void for_overlap_pairs(const kv::Tensor& A, const kv::Tensor& B, F&& f) {
+    for (const auto& ba : A.lay.blocks) {
+        const int j = B.lay.find(ba.qn);
+        if (j < 0 || !kv::qn_equal(B.lay.blocks[j].shape, ba.shape, A.lay.rank)) continue;
+        f(kv::DotPair{B.lay.blocks[j].off, ba.off, ba.size});
+    }
+}
+
 kv::Tensor* need(const kv_tensor* t, const char* what) {
     KV_REQUIRE(t && t->t, kv::ERR_ARG, std::string("null tensor: ") + what);
     return t->t.get();
@@ -200,22 +211,19 @@ int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im) {
         kv::Tensor* A = need(a, "a");
         kv::Tensor* B = need(b, "b");
         kv::Device& D = *a->ctx->dev;
-        // sum over blocks of equal qn and shape of <b|a>
-        kv::DevBuf s(D.pool.get(), 16), scr(D.pool.get(), 2 * 1024 * sizeof(double));
-        double acc[2] = {0, 0};
-        for (const auto& ba : A->lay.blocks) {
-            const int j = B->lay.find(ba.qn);
-            if (j < 0 || !kv::qn_equal(B->lay.blocks[j].shape, ba.shape, A->lay.rank)) continue;
-            kv::launch_dotc(D.stream, B->d() + B->lay.blocks[j].off, A->d() + ba.off, ba.size,
-                            s.as<double2>(), scr.as<double>());
-            double h[2];
-            KV_HIP(hipMemcpyAsync(h, s.get(), 16, hipMemcpyDeviceToHost, D.stream));
-            D.sync();
-            acc[0] += h[0];
-            acc[1] += h[1];
-        }
-        re_im[0] = acc[0];
-        re_im[1] = acc[1];
+        // <b|a>: one reduction over every pair of blocks of equal qn and shape
+        std::vector<kv::DotPair> pairs;
+        int64_t total = 0;
+        for_overlap_pairs(*A, *B, [&](const kv::DotPair& p) {
+            pairs.push_back(p);
+            total += p.size;
+        });
+        kv::DevBuf dp = kv::upload_vec(D, pairs);
+        kv::DevBuf part(D.pool.get(), 2 * kv::kRedBlocks * sizeof(double)), out(D.pool.get(), 16);
+        kv::launch_dotc_pairs(D.stream, B->d(), A->d(), dp.as<kv::DotPair>(), (int)pairs.size(), total,
+                              part.as<double>(), out.as<double2>());
+        KV_HIP(hipMemcpyAsync(re_im, out.get(), 16, hipMemcpyDeviceToHost, D.stream));
+        D.sync();
     });
 }
 
@@ -481,7 +489,8 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
     const kv::KernelStats& h = s->s->heff_kernels();
     const kv::KernelStats& e = s->s->env_kernels();
     const kv::TrdStats& r = s->s->trd_stats();
-    const double v[25] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
+    const kv::BlockStats& bk = s->s->device().blk;
+    const double v[35] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
                           (h.gemm_ms + h.wfold_ms) * 1e-3, h.gemm_flop + h.wfold_flop,
                           (e.gemm_ms + e.wfold_ms) * 1e-3, e.gemm_flop + e.wfold_flop,
                           (double)s->s->lanczos_stats().matvecs,
@@ -489,8 +498,11 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
                           h.wfold_ms * 1e-3, h.wfold_flop, (double)h.wfold_launches,
                           h.gemm_bytes, h.gemm32_ms * 1e-3, h.gemm32_flop,
                           (double)h.gemm32_launches, h.gemm32_bytes,
-                          r.ms * 1e-3, r.bytes, (double)r.columns};
-    for (int i = 0; i < n && i < 25; ++i) out[i] = v[i];
+                          r.ms * 1e-3, r.bytes, (double)r.columns,
+                          bk.gemm_ms * 1e-3, bk.gemm_flop, (double)bk.gemm_launches, bk.gemm_bytes,
+                          bk.orth_ms * 1e-3, (double)bk.orth_calls, bk.eig_ms * 1e-3,
+                          (double)bk.eig_calls, (double)bk.sectors, (double)bk.fallbacks};
+    for (int i = 0; i < n && i < 35; ++i) out[i] = v[i];
     return KV_OK;
 }
 

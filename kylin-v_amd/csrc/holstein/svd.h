@@ -74,6 +74,7 @@ struct SvdResult {
     int total = 0;                           // singular values before truncation
     std::vector<std::vector<double>> svals;  // per sector pair, kept values
     int gram_sectors = 0, exact_sectors = 0, refined_sectors = 0;
+    int block_sectors = 0, block_fallbacks = 0;  // block subspace route: kept, redone in full
     std::string detail;  // per sector: rows x cols, route, s_max, last kept value
 };
 
@@ -85,6 +86,7 @@ constexpr int kSplitEigMin = 1024;
 enum EigMode { EIG_AUTO = 0, EIG_SPLIT = 1, EIG_HEEVD = 2 };
 
 struct SvdOptions {
+    bool block = true;      // block subspace route for sectors far larger than the kept dimension
     double guard = 1e-4;    // smaller-side Gram vectors only if s_kept_min >= guard * s_max
     int iso_min = 64;       // smaller side from which the Gram routes are used
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
@@ -94,7 +96,10 @@ struct SvdOptions {
     static SvdOptions from_env() {
         SvdOptions o;
         if (const char* e = std::getenv("KV_TRD")) o.roc_trd = std::string(e) == "rocsolver";
-        if (const char* e = std::getenv("KV_SVD")) o.force_exact = std::string(e) == "exact";
+        if (const char* e = std::getenv("KV_SVD")) {
+            o.force_exact = std::string(e) == "exact";
+            o.block = std::string(e) != "full" && !o.force_exact;
+        }
         if (const char* e = std::getenv("KV_EIG"))
             o.eig_mode = std::string(e) == "heevd" ? EIG_HEEVD : std::string(e) == "split" ? EIG_SPLIT : EIG_AUTO;
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
@@ -149,6 +154,12 @@ struct SectorSvd {
     // from which gram_eigvecs forms the rows on demand.
     DevBuf Ev, tau, Zt;
     int ev_lo = 0;
+    // block subspace route: Z (isometric side x kc) the leading Ritz vectors, W (other side
+    // x kc) the sector matrix applied to them (it carries S)
+    bool block = false;
+    DevBuf Z, W;
+    int kc = 0;
+    double ceil = 0.0;  // every singular value beyond the certified ones is at most this
 };
 
 inline void exact_sector(Device& D, Solver& sol, SectorSvd& X, bool keep_zero) {
@@ -717,6 +728,298 @@ inline void iso_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
     D.sync();
 }
 
+// ---- block subspace route -------------------------------------------------------------
+// A two-site sector matrix of a D = 256 bond has sides in the thousands (2048 x 4096 for the
+// 32-site chain) but keeps at most maxdim values.  Its leading singular subspace on the
+// isometric side comes from a seeded random start block of maxdim + kBlockOversample
+// columns with one power step, orthonormalised between the products (randomised range
+// finder with subspace iteration: Halko, Martinsson and Tropp, SIAM Review 53 (2011),
+// algorithms 4.3/4.4), then Rayleigh-Ritz on the block.  Each Ritz pair's residual
+// |G z - theta z| (G the Gram matrix of the isometric side) is measured, and only the leading
+// pairs whose residual is at most kBlockResid x the largest Ritz value count as singular
+// values.  A sector whose truncation would keep every such value, or reach below the tail
+// threshold where the Gram eigenvalues lose accuracy, is redone by the full eigensolver.
+constexpr int kBlockOversample = 32;
+constexpr double kBlockDrop = 1e-12;   // Cholesky QR pivot threshold (x the largest)
+constexpr double kBlockResid = 1e-13;  // Ritz residual bound (x the largest Ritz value)
+constexpr int kBlockPowerMax = 3;      // power steps before the sector goes to the full route
+// rounding allowance of |M|_F^2 - sum of certified theta (relative to |M|_F^2)
+constexpr double kBlockMassSlack = 1e-13;
+
+// C (m x n, ld n) = op(A) op(B).  When the output has too few 64 x 64 tiles to fill the GPU,
+// K is cut into chunks whose partial products a second kernel sums in chunk order.
+inline void gemm_chunked(Device& D, char opA, char opB, const double2* A, int lda, const double2* B, int ldb,
+                         double2* C, int m, int n, int k) {
+    const int tiles = ((m + GEMM_BM - 1) / GEMM_BM) * ((n + GEMM_BN - 1) / GEMM_BN);
+    int nch = 1;
+    while (tiles * nch < 512 && k / (2 * nch) >= 256) nch *= 2;
+    EventTimer ev;
+    ev.start(D.stream);
+    D.blk.gemm_flop += 8.0 * m * n * (double)k;
+    D.blk.gemm_bytes += 16.0 * ((double)m * k + (double)k * n + (double)m * n);
+    D.blk.gemm_launches += 1;
+    if (nch == 1) {
+        gemm_dense(D, opA, opB, A, lda, B, ldb, C, n, m, n, k);
+        ev.stop(D.stream);
+        D.blk.gemm_ms += ev.ms();
+        return;
+    }
+    const int kc = ((k + nch - 1) / nch + 7) / 8 * 8;
+    nch = (k + kc - 1) / kc;
+    DevBuf part(D.pool.get(), (size_t)nch * m * n * sizeof(double2));
+    std::vector<GemmGroup> gs;
+    std::vector<GemmSeg> ss;
+    std::vector<GemmTile> ts;
+    for (int c = 0; c < nch; ++c) {
+        const int k0 = c * kc;
+        GemmGroup g{};
+        g.c_off = (int64_t)c * m * n;
+        g.m = m;
+        g.n = n;
+        g.ldc = n;
+        g.seg_begin = c;
+        g.seg_count = 1;
+        GemmSeg sg{};
+        sg.a_off = opA == 'N' ? (int64_t)k0 : (int64_t)k0 * lda;
+        sg.b_off = opB == 'N' ? (int64_t)k0 * ldb : (int64_t)k0;
+        sg.k = std::min(kc, k - k0);
+        sg.lda = lda;
+        sg.ldb = ldb;
+        gs.push_back(g);
+        ss.push_back(sg);
+        for (int a = 0; a < (m + GEMM_BM - 1) / GEMM_BM; ++a)
+            for (int b = 0; b < (n + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{c, a, b, 0});
+    }
+    DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
+    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, part.as<double2>(), dt.as<GemmTile>(),
+                         (int)ts.size(), dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
+    launch_sum_parts(D.stream, part.as<double2>(), nch, (int64_t)m * n, C);
+    ev.stop(D.stream);
+    D.blk.gemm_ms += ev.ms();
+    D.sync();
+}
+
+// Orthonormal basis of the columns of Y (n x c, row-major, ld c) by two Cholesky QR passes,
+// Q = Y R^{-1} with R from Y^H Y.  Columns whose pivot is at most kBlockDrop x the largest
+// are dependent on the earlier ones and are dropped, so the basis has r <= c columns
+// (returned, n x r, ld r).  One pass leaves an orthogonality error ~eps cond(Y)^2 <= 1e-4
+// on the kept columns; the second brings it to rounding.
+inline DevBuf block_orth(Device& D, DevBuf Y, int n, int c, int& r) {
+    int cc = c;
+    for (int pass = 0; pass < 2 && cc > 0; ++pass) {
+        KV_REQUIRE(cc <= kCholMax, ERR_INTERNAL, "block route: block wider than the Cholesky kernel");
+        const size_t cc2 = (size_t)cc * cc;
+        DevBuf S(D.pool.get(), cc2 * sizeof(double2)), T(D.pool.get(), cc2 * sizeof(double2));
+        DevBuf live(D.pool.get(), (size_t)cc * sizeof(int));
+        DevBuf sz = upload_vec(D, std::vector<int>{cc});
+        gemm_chunked(D, 'C', 'N', Y.as<double2>(), cc, Y.as<double2>(), cc, S.as<double2>(), cc, cc, n);
+        EventTimer ev;
+        ev.start(D.stream);
+        launch_chol_upper(D.stream, S.as<double2>(), sz.as<int>(), cc, (int64_t)cc2, kBlockDrop, live.as<int>(), 1);
+        launch_tri_inv(D.stream, S.as<double2>(), T.as<double2>(), sz.as<int>(), live.as<int>(), cc, (int64_t)cc2,
+                       cc, 1);
+        ev.stop(D.stream);
+        D.blk.orth_ms += ev.ms();
+        D.blk.orth_calls += 1;
+        std::vector<int> hl(cc);
+        KV_HIP(hipMemcpyAsync(hl.data(), live.get(), (size_t)cc * sizeof(int), hipMemcpyDeviceToHost, D.stream));
+        DevBuf Q(D.pool.get(), (size_t)n * cc * sizeof(double2));
+        gemm_chunked(D, 'N', 'N', Y.as<double2>(), cc, T.as<double2>(), cc, Q.as<double2>(), n, cc, cc);
+        int nl = 0;
+        for (int v : hl) nl += v ? 1 : 0;
+        if (nl < cc && nl > 0) {  // keep the live columns, in order, as runs of adjacent columns
+            std::vector<Copy2D> cps;
+            int dst = 0;
+            for (int j = 0; j < cc;) {
+                if (!hl[j]) {
+                    ++j;
+                    continue;
+                }
+                int e = j;
+                while (e < cc && hl[e]) ++e;
+                Copy2D cp{};
+                cp.src_off = j;
+                cp.dst_off = dst;
+                cp.rows = n;
+                cp.cols = e - j;
+                cp.src_ld = cc;
+                cp.dst_ld = nl;
+                cps.push_back(cp);
+                dst += e - j;
+                j = e;
+            }
+            DevBuf Qc(D.pool.get(), (size_t)n * nl * sizeof(double2));
+            DevBuf dc = upload_vec(D, cps);
+            launch_copy2d<double2>(D.stream, Q.as<double2>(), Qc.as<double2>(), dc.as<Copy2D>(), (int)cps.size(), n,
+                                   nullptr);
+            D.sync();
+            Q = std::move(Qc);
+        }
+        Y = std::move(Q);
+        cc = nl;
+    }
+    r = cc;
+    return Y;
+}
+
+// The block route for one sector with the isometry on its rows (iso_rows, S going right) or
+// its columns: A = M or M^H so that A's rows are the isometric side (ni x no).
+//   Y = A Omega (Omega no x b seeded uniform),  Q = orth(Y),  then per power step
+//   P = orth(A^H Q),  Q = orth(A P),  P = A^H Q,  H = P^H P = Q^H A A^H Q = V Theta V^H,
+//   Z = Q V (Ritz vectors),  W = A^H Z = P V,  residual_j = |A W_j - theta_j Z_j|,
+// repeated (at most kBlockPowerMax steps) until the leading min(r, kmax) pairs all pass.
+// X.s receives the leading singular values sqrt(theta_j) whose residuals pass, X.Z and X.W
+// the leading Ritz vectors and their products, X.ceil the bound on every other value.
+// Returns false if no pair passes.
+inline bool block_sector(Device& D, Solver& sol, SectorSvd& X, bool iso_rows, int b, int kmax, uint64_t seed,
+                         int mode, bool roc_trd) {
+    const int nr = (int)X.nr, nc = (int)X.nc;
+    const int ni = iso_rows ? nr : nc, no = iso_rows ? nc : nr;
+    const double2* m = X.M.as<double2>();
+    auto amul = [&](const double2* Bm, int c, double2* out) {  // out (ni x c) = A Bm
+        if (iso_rows)
+            gemm_chunked(D, 'N', 'N', m, nc, Bm, c, out, nr, c, nc);
+        else
+            gemm_chunked(D, 'C', 'N', m, nc, Bm, c, out, nc, c, nr);
+    };
+    auto amul_h = [&](const double2* Qm, int c, double2* out) {  // out (no x c) = A^H Qm
+        if (iso_rows)
+            gemm_chunked(D, 'C', 'N', m, nc, Qm, c, out, nc, c, nr);
+        else
+            gemm_chunked(D, 'N', 'N', m, nc, Qm, c, out, nr, c, nc);
+    };
+    DevBuf Y(D.pool.get(), (size_t)ni * b * sizeof(double2));
+    {
+        DevBuf Om(D.pool.get(), (size_t)no * b * sizeof(double2));
+        launch_rand_fill(D.stream, Om.as<double2>(), (int64_t)no * b, seed);
+        amul(Om.as<double2>(), b, Y.as<double2>());
+    }
+    int r = 0;
+    DevBuf Q = block_orth(D, std::move(Y), ni, b, r);
+    if (r == 0) return false;
+    double f2 = 0.0;
+    {
+        DevBuf sc(D.pool.get(), 1024 * sizeof(double)), nr2(D.pool.get(), sizeof(double));
+        launch_nrm2sq(D.stream, m, X.nr * X.nc, nr2.as<double>(), sc.as<double>());
+        KV_HIP(hipMemcpyAsync(&f2, nr2.get(), sizeof(double), hipMemcpyDeviceToHost, D.stream));
+        D.sync();
+    }
+    const double gscale = f2 > 0 ? f2 : 1.0;
+    DevBuf P;
+    std::vector<double> theta;
+    int kc = 0, cert = 0;
+    for (int q = 1; q <= kBlockPowerMax; ++q) {
+        // one power step, orthonormalised on both sides: Q <- orth(A orth(A^H Q))
+        P = DevBuf(D.pool.get(), (size_t)no * r * sizeof(double2));
+        amul_h(Q.as<double2>(), r, P.as<double2>());
+        int rp = 0;
+        P = block_orth(D, std::move(P), no, r, rp);
+        if (rp == 0) return false;
+        Q = DevBuf(D.pool.get(), (size_t)ni * rp * sizeof(double2));
+        amul(P.as<double2>(), rp, Q.as<double2>());
+        Q = block_orth(D, std::move(Q), ni, rp, r);
+        if (r == 0) return false;
+        P = DevBuf(D.pool.get(), (size_t)no * r * sizeof(double2));
+        amul_h(Q.as<double2>(), r, P.as<double2>());
+        // Rayleigh-Ritz at unit scale (the eigensolver sees the same scale as in the full route)
+        SectorSvd H;
+        H.k = r;
+        H.gram = true;
+        H.G = DevBuf(D.pool.get(), (size_t)r * r * sizeof(double2));
+        gemm_chunked(D, 'C', 'N', P.as<double2>(), r, P.as<double2>(), r, H.G.as<double2>(), r, r, no);
+        launch_zaxpby(D.stream, H.G.as<double2>(), H.G.as<double2>(), nullptr, make_double2(1.0 / gscale, 0.0),
+                      make_double2(0.0, 0.0), (int64_t)r * r);
+        {
+            EventTimer ev;
+            ev.start(D.stream);
+            gram_solve(D, sol, {&H}, mode, roc_trd);
+            gram_eigvecs(D, sol, H, 0);
+            ev.stop(D.stream);
+            D.blk.eig_ms += ev.ms();
+            D.blk.eig_calls += 1;
+        }
+        kc = std::min(r, kmax);
+        DevBuf Vd(D.pool.get(), (size_t)kc * r * sizeof(double2));  // rows v_j^H, descending
+        {
+            std::vector<Copy2D> cps(kc);
+            for (int j = 0; j < kc; ++j) {
+                Copy2D c{};
+                c.src_off = (int64_t)(r - 1 - j - H.ev_lo) * r;
+                c.dst_off = (int64_t)j * r;
+                c.rows = 1;
+                c.cols = r;
+                c.src_ld = r;
+                c.dst_ld = r;
+                cps[j] = c;
+            }
+            DevBuf dc = upload_vec(D, cps);
+            launch_copy2d<double2>(D.stream, H.Ev.as<double2>(), Vd.as<double2>(), dc.as<Copy2D>(), kc, 1, nullptr);
+        }
+        X.Z = DevBuf(D.pool.get(), (size_t)ni * kc * sizeof(double2));
+        X.W = DevBuf(D.pool.get(), (size_t)no * kc * sizeof(double2));
+        gemm_chunked(D, 'N', 'C', Q.as<double2>(), r, Vd.as<double2>(), r, X.Z.as<double2>(), ni, kc, r);
+        gemm_chunked(D, 'N', 'C', P.as<double2>(), r, Vd.as<double2>(), r, X.W.as<double2>(), no, kc, r);
+        // residuals of the Ritz pairs: A W_j = G z_j against theta_j z_j
+        theta.assign(kc, 0.0);
+        for (int j = 0; j < kc; ++j) theta[j] = H.s[j] * H.s[j] * gscale;
+        DevBuf GZ(D.pool.get(), (size_t)ni * kc * sizeof(double2));
+        amul(X.W.as<double2>(), kc, GZ.as<double2>());
+        constexpr int rows_per = 256;
+        const int chunks = (ni + rows_per - 1) / rows_per;
+        DevBuf dth = upload_vec(D, theta);
+        DevBuf part(D.pool.get(), (size_t)chunks * kc * sizeof(double));
+        launch_resid_cols(D.stream, GZ.as<double2>(), kc, X.Z.as<double2>(), kc, dth.as<double>(), ni, kc,
+                          rows_per, part.as<double>());
+        std::vector<double> hp((size_t)chunks * kc);
+        KV_HIP(hipMemcpyAsync(hp.data(), part.get(), hp.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+        D.sync();
+        const double bound = kBlockResid * theta[0];
+        cert = 0;
+        for (; cert < kc; ++cert) {
+            double r2s = 0.0;
+            for (int y = 0; y < chunks; ++y) r2s += hp[(size_t)y * kc + cert];
+            if (!(std::sqrt(r2s) <= bound)) break;
+        }
+        if (cert == kc) break;
+    }
+    if (cert == 0) {
+        X.Z.reset();
+        X.W.reset();
+        return false;
+    }
+    // the sector's Frobenius mass outside the certified pairs bounds every other value
+    double mass = f2;
+    for (int j = 0; j < cert; ++j) mass -= theta[j];
+    X.ceil = std::sqrt(std::max(mass, 0.0) + kBlockMassSlack * f2);
+    X.s.assign((size_t)cert, 0.0);
+    for (int j = 0; j < cert; ++j) X.s[j] = std::sqrt(theta[j]);
+    X.block = true;
+    X.iso = true;
+    X.gram = false;
+    X.colside = !iso_rows;
+    X.kc = kc;
+    X.k = cert;
+    return true;
+}
+
+// Factors of a block-route sector: the isometry is Z's leading columns, the other factor W
+// (which already carries S), conjugate-transposed where the layout needs rows.
+inline void block_vectors(Device& D, SectorSvd& X, int nk) {
+    const int kc = X.kc;
+    X.s_absorbed = true;
+    X.S = upload_vec(D, std::vector<double>(X.s.begin(), X.s.begin() + nk));
+    X.ldU = kc;
+    X.Vh = DevBuf(D.pool.get(), (size_t)kc * X.nc * sizeof(double2));
+    if (!X.colside) {  // U = Z (nr x kc); S V^H = W^H (kc x nc)
+        launch_ctranspose(D.stream, X.W.as<double2>(), X.Vh.as<double2>(), (int)X.nc, kc, kc, (int)X.nc);
+        X.U = std::move(X.Z);
+    } else {  // V^H = Z^H (kc x nc); U S = M V = W (nr x kc)
+        launch_ctranspose(D.stream, X.Z.as<double2>(), X.Vh.as<double2>(), (int)X.nc, kc, kc, (int)X.nc);
+        X.U = std::move(X.W);
+    }
+    D.sync();
+}
+
 inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, const Tensor& A,
                               int r1, double cutoff, int maxdim, bool left2right,
                               int left_dir = -1, const SvdOptions& opt = SvdOptions::from_env()) {
@@ -759,6 +1062,16 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
                                    (int)cps.size(), maxrows, nullptr);
             D.sync();
         }
+        // sectors at least twice as wide as the block take the block subspace route
+        const int bw = maxdim + kBlockOversample;
+        const bool blk = opt.block && !opt.force_exact && cutoff >= 0.0 && maxdim > 0 && bw <= kCholMax &&
+                         std::min(rs.len, cs.len) >= 2 * (int64_t)bw;
+        if (blk) {
+            const uint64_t seed = ((uint64_t)rs.len << 40) ^ ((uint64_t)cs.len << 16) ^ (uint64_t)p;
+            if (block_sector(D, sol, X[p], left2right, bw, maxdim + 1, seed, opt.eig_mode, opt.roc_trd)) continue;
+            res.block_fallbacks += 1;
+            D.blk.fallbacks += 1;
+        }
         const int64_t iso_side = left2right ? rs.len : cs.len;
         const int64_t oth_side = left2right ? cs.len : rs.len;
         const bool big = !opt.force_exact && std::min(rs.len, cs.len) >= opt.iso_min;
@@ -784,24 +1097,46 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             gram_solve(D, sol, all, opt.eig_mode, opt.roc_trd);
         // the isometric route's values: only min(rows, cols) are singular values
         for (int p = 0; p < np; ++p)
-            if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
+            if (X[p].iso && !X[p].block) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
     }
     // phase 2: global truncation; Gram sectors that would keep values too small for the
     // Gram eigenvalues' absolute accuracy move to the exact route and the rule reruns
     std::vector<int> kept;
-    // each sector changes route at most once and is refined at most once, so the rule
-    // settles within four passes
-    for (int pass = 0; pass < 4; ++pass) {
+    // every redo moves one sector forward through block -> full isometric route -> refined
+    // tail (or smaller-side Gram -> isometric), so the rule settles
+    for (int pass = 0;; ++pass) {
+        KV_REQUIRE(pass <= 4 * np + 4, ERR_INTERNAL, "bond truncation did not settle");
         std::vector<std::vector<double>> sv(np);
         for (int p = 0; p < np; ++p) sv[p] = X[p].s;
         kept = truncation_counts(sv, cutoff, maxdim);
         // smallest kept value overall: an isometric-route tail that may reach it is refined
         double s_last = -1.0;
-        for (int p = 0; p < np; ++p)
+        int tot = 0;
+        for (int p = 0; p < np; ++p) {
+            tot += kept[p];
             if (kept[p] > 0 && (s_last < 0 || X[p].s[kept[p] - 1] < s_last)) s_last = X[p].s[kept[p] - 1];
+        }
         bool redo = false;
         for (int p = 0; p < np; ++p) {
-            if (X[p].iso) {
+            if (X[p].block) {
+                // the cut must stay above the tail, and either fall inside the certified
+                // values or leave every value beyond them (at most ceil) out as well
+                const size_t ns = X[p].s.size();
+                bool ok = !(kept[p] > 0 && X[p].s[kept[p] - 1] < opt.tail * X[p].s[0]);
+                if ((size_t)kept[p] >= ns)
+                    ok = ok && (X[p].ceil <= cutoff || (tot >= maxdim && X[p].ceil < s_last));
+                if (!ok) {
+                    X[p].block = false;
+                    X[p].Z.reset();
+                    X[p].W.reset();
+                    gram_prepare(D, X[p], true, !left2right);
+                    gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
+                    X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
+                    res.block_fallbacks += 1;
+                    D.blk.fallbacks += 1;
+                    redo = true;
+                }
+            } else if (X[p].iso) {
                 if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
                     opt.tail * X[p].s[0] >= s_last) {
                     iso_refine(D, sol, X[p], opt.tail, opt.eig_mode, opt.roc_trd);
@@ -824,11 +1159,15 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         char buf[160];
         std::snprintf(buf, sizeof buf, " [%lldx%lld %s smax %.2e kept %d last %.2e]",
                       (long long)X[p].nr, (long long)X[p].nc,
-                      X[p].iso ? (X[p].refined ? "iso2" : "iso") : (X[p].gram ? "gram" : "exact"),
+                      X[p].block ? "block" : X[p].iso ? (X[p].refined ? "iso2" : "iso") : (X[p].gram ? "gram" : "exact"),
                       X[p].s.empty() ? 0.0 : X[p].s[0], kept[p],
                       kept[p] > 0 ? X[p].s[kept[p] - 1] : 0.0);
         res.detail += buf;
-        if (X[p].iso) {
+        if (X[p].block) {
+            res.block_sectors += 1;
+            D.blk.sectors += 1;
+            if (kept[p] > 0) block_vectors(D, X[p], kept[p]);
+        } else if (X[p].iso) {
             res.gram_sectors += 1;
             if (kept[p] > 0) iso_vectors(D, sol, X[p], kept[p]);
         } else if (X[p].gram) {

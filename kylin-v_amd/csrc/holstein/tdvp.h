@@ -233,6 +233,7 @@ class TDVP {
         heff_k_ = KernelStats();
         env_k_ = KernelStats();
         dev_->trd = TrdStats();
+        dev_->blk = BlockStats();
     }
     const TrdStats& trd_stats() const { return dev_->trd; }
     bool timing = true;

@@ -93,4 +93,26 @@ void launch_trd_column(hipStream_t st, double2* A, double2* VW, const double2* y
 void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, double2* y, double2* q, int n, int s,
                        int i, int nb, int batch);
 
+// Block subspace route of the bond SVD.
+// p[e] = uniform [-1, 1)^2 complex value determined by (seed, e).
+void launch_rand_fill(hipStream_t st, double2* p, int64_t n, uint64_t seed);
+// Largest side of the Cholesky / triangular-inverse kernels.
+constexpr int kCholMax = 1024;
+// In place upper Cholesky factor of `batch` row-major Hermitian matrices (side sizes[b], ld,
+// stride); pivots <= tau x the largest diagonal drop their column (live[b ld + k] = 0).
+void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
+                       int batch);
+// T = R^{-1} over the live columns (dropped rows and columns zero); nmax = largest side.
+void launch_tri_inv(hipStream_t st, const double2* R, double2* T, const int* sizes, const int* live, int ld,
+                    int64_t stride, int nmax, int batch);
+// part[y kc + j] = sum over row chunk y of |G[r][j] - theta_j Z[r][j]|^2.
+void launch_resid_cols(hipStream_t st, const double2* G, int ldg, const double2* Z, int ldz, const double* theta,
+                       int n, int kc, int rows_per, double* part);
+
+// out[0] = sum over the pairs of conj(x) y (double2); part holds 2 x kRedBlocks doubles of
+// partial sums.  Deterministic: a fixed element-to-block assignment and a fixed final order.
+constexpr int kRedBlocks = 1024;
+void launch_dotc_pairs(hipStream_t st, const double2* x, const double2* y, const DotPair* pairs, int npairs,
+                       int64_t total, double* part, double2* out);
+
 }  // namespace kv

@@ -716,7 +716,7 @@ void launch_ctranspose(hipStream_t st, const double2* src, double2* dst, int row
 // ------------------------------------------------------------------------------------
 // FP64 vector algebra
 // ------------------------------------------------------------------------------------
-constexpr int RED_BLOCKS = 1024;
+constexpr int RED_BLOCKS = kRedBlocks;
 
 __global__ __launch_bounds__(256) void zero_kernel(uint32_t* __restrict__ p, int64_t n) {
     for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
@@ -1480,6 +1480,244 @@ void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, doub
     const int ntask = n - s + 2 * i;
     hipLaunchKernelGGL(trd_matvec_kernel, dim3((ntask + 1) / 2, batch), dim3(256), 0, st, A, VW, y, q, n, s, i,
                        nb);
+    KV_HIP(hipGetLastError());
+}
+
+// ---- block subspace route of the bond SVD --------------------------------------------
+// Randomised range finding on a sector matrix (Halko, Martinsson and Tropp, SIAM Review 53
+// (2011), algorithms 4.3/4.4 with Rayleigh-Ritz): a seeded random start block, Cholesky QR
+// orthonormalisation with rank-deficient columns dropped, and the Ritz residual check.
+
+// splitmix64 output function (Steele, Lea and Flood, OOPSLA 2014): a bijective 64-bit mix.
+__device__ inline uint64_t mix64(uint64_t z) {
+    z += 0x9e3779b97f4a7c15ull;
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    return z ^ (z >> 31);
+}
+
+// p[e] = (u, v) uniform in [-1, 1)^2, a function of (seed, e) only.
+__global__ __launch_bounds__(256) void rand_fill_kernel(double2* __restrict__ p, int64_t n, uint64_t seed) {
+    const uint64_t s = mix64(seed);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const uint64_t a = mix64(s ^ (uint64_t)(2 * e)), b = mix64(s ^ (uint64_t)(2 * e + 1));
+        p[e] = make_double2((double)(a >> 11) * 0x1.0p-52 - 1.0, (double)(b >> 11) * 0x1.0p-52 - 1.0);
+    }
+}
+
+void launch_rand_fill(hipStream_t st, double2* p, int64_t n, uint64_t seed) {
+    if (n <= 0) return;
+    const int blocks = (int)std::min<int64_t>((n + 255) / 256, 4096);
+    hipLaunchKernelGGL(rand_fill_kernel, dim3(blocks), dim3(256), 0, st, p, n, seed);
+    KV_HIP(hipGetLastError());
+}
+
+__device__ inline double wave_max(double v) {
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Upper Cholesky factor S = R^H R of row-major Hermitian matrices (the upper triangle is
+// read), one 1024-thread workgroup per matrix, right-looking, in place.  A pivot at or below
+// tau x the largest diagonal entry marks its column dependent: its row of R is zero and it
+// takes no part in the elimination, so R is the factor of S restricted to the live columns.
+// live[b ld + k] = 1 for a live column.  n = sizes[b] <= kCholMax.
+__global__ __launch_bounds__(1024) void chol_upper_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
+                                                          int ld, int64_t stride, double tau,
+                                                          int* __restrict__ live) {
+    const int b = blockIdx.x, n = sizes[b];
+    S += (int64_t)b * stride;
+    live += (int64_t)b * ld;
+    __shared__ double2 rk[kCholMax];
+    __shared__ double red[16];
+    __shared__ double dmax_s, rinv_s;
+    __shared__ int live_s;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    double m = 0.0;
+    for (int i = tid; i < n; i += 1024) m = fmax(m, S[(int64_t)i * ld + i].x);
+    m = wave_max(m);
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < 16; ++w) t = fmax(t, red[w]);
+        dmax_s = t;
+    }
+    __syncthreads();
+    const double thr = tau * dmax_s;
+    for (int k = 0; k < n; ++k) {
+        if (tid == 0) {
+            const double piv = S[(int64_t)k * ld + k].x;
+            const int lv = (piv > thr && piv > 0.0) ? 1 : 0;  // NaN is not live
+            live_s = lv;
+            rinv_s = lv ? 1.0 / sqrt(piv) : 0.0;
+            live[k] = lv;
+        }
+        __syncthreads();
+        const double ri = rinv_s;
+        const int lv = live_s;
+        for (int j = k + tid; j < n; j += 1024) {
+            double2 v = S[(int64_t)k * ld + j];
+            v = (j == k) ? make_double2(lv ? 1.0 / ri : 0.0, 0.0) : make_double2(v.x * ri, v.y * ri);
+            rk[j] = v;
+            S[(int64_t)k * ld + j] = v;
+        }
+        __syncthreads();
+        if (lv) {  // S_ij -= conj(R_ki) R_kj, k < i <= j
+            for (int i = k + 1 + wv; i < n; i += 16) {
+                const double2 a = rk[i];
+                for (int j = i + lane; j < n; j += 64) {
+                    const double2 c = rk[j];
+                    double2 x = S[(int64_t)i * ld + j];
+                    x.x -= a.x * c.x + a.y * c.y;
+                    x.y -= a.x * c.y - a.y * c.x;
+                    S[(int64_t)i * ld + j] = x;
+                }
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// T = R^{-1} over the live columns of an upper factor from chol_upper_kernel (row-major, rows
+// and columns of dropped indices zero).  One wave per column c: back substitution
+// T_ic = -(sum_{i<k<=c} R_ik T_kc) / R_ii from i = c down, with column c kept in LDS and the
+// next row of R loaded while the current one is reduced.  grid (ceil(nmax / 4), batch).
+__global__ __launch_bounds__(256) void tri_inv_kernel(const double2* __restrict__ R, double2* __restrict__ T,
+                                                      const int* __restrict__ sizes,
+                                                      const int* __restrict__ live, int ld, int64_t stride) {
+    const int b = blockIdx.y, n = sizes[b];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int c = blockIdx.x * 4 + wv;
+    R += (int64_t)b * stride;
+    T += (int64_t)b * stride;
+    live += (int64_t)b * ld;
+    __shared__ double2 xs[4][kCholMax];
+    if (c >= n) return;  // c is uniform over the wave, and nothing below synchronises the workgroup
+    double2* x = xs[wv];
+    const bool lc = live[c] != 0;
+    constexpr int U = kCholMax / 64;
+    double2 rn[U];  // row i of R, entries k = i + 1 + lane + 64u
+    auto load_row = [&](int i) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = i + 1 + lane + 64 * u;
+            rn[u] = (k <= c) ? R[(int64_t)i * ld + k] : make_double2(0.0, 0.0);
+        }
+    };
+    if (lc) load_row(c);
+    for (int i = c; i >= 0; --i) {
+        double2 val = make_double2(0.0, 0.0);
+        if (lc && live[i]) {
+            double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int k = i + 1 + lane + 64 * u;
+                if (k <= c) {
+                    const double2 r = rn[u], t = x[k];
+                    acc.x += r.x * t.x - r.y * t.y;
+                    acc.y += r.x * t.y + r.y * t.x;
+                }
+            }
+            const double rii = R[(int64_t)i * ld + i].x;
+            if (i > 0) load_row(i - 1);
+            acc = wave_sum2(acc);
+            val = (i == c) ? make_double2(1.0 / rii, 0.0) : make_double2(-acc.x / rii, -acc.y / rii);
+        } else if (lc && i > 0) {
+            load_row(i - 1);
+        }
+        x[i] = val;  // every lane holds the same value (butterfly sum); the wave's LDS ops stay in order
+        __builtin_amdgcn_wave_barrier();
+    }
+    for (int i = lane; i < n; i += 64) T[(int64_t)i * ld + c] = (i <= c) ? x[i] : make_double2(0.0, 0.0);
+}
+
+void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
+                       int batch) {
+    KV_REQUIRE(ld <= kCholMax && batch >= 1, ERR_INTERNAL, "chol_upper: side above kCholMax");
+    hipLaunchKernelGGL(chol_upper_kernel, dim3(batch), dim3(1024), 0, st, S, sizes, ld, stride, tau, live);
+    KV_HIP(hipGetLastError());
+}
+
+void launch_tri_inv(hipStream_t st, const double2* R, double2* T, const int* sizes, const int* live, int ld,
+                    int64_t stride, int nmax, int batch) {
+    KV_REQUIRE(ld <= kCholMax && nmax <= ld && batch >= 1, ERR_INTERNAL, "tri_inv: side above kCholMax");
+    if (nmax <= 0) return;
+    hipLaunchKernelGGL(tri_inv_kernel, dim3((nmax + 3) / 4, batch), dim3(256), 0, st, R, T, sizes, live, ld, stride);
+    KV_HIP(hipGetLastError());
+}
+
+// part[y kc + j] = sum over rows [y rows_per, (y + 1) rows_per) of |G[r][j] - theta_j Z[r][j]|^2
+// (row-major G and Z, ld ldg and ldz).  grid (ceil(kc / 64), chunks), 4 row slices per block.
+__global__ __launch_bounds__(256) void resid_cols_kernel(const double2* __restrict__ G, int ldg,
+                                                         const double2* __restrict__ Z, int ldz,
+                                                         const double* __restrict__ theta, int n, int kc,
+                                                         int rows_per, double* __restrict__ part) {
+    const int l = threadIdx.x & 63, sl = threadIdx.x >> 6;
+    const int j = blockIdx.x * 64 + l;
+    const int r0 = blockIdx.y * rows_per, r1 = min(n, r0 + rows_per);
+    __shared__ double red[4][64];
+    double s = 0.0;
+    if (j < kc) {
+        const double t = theta[j];
+        for (int r = r0 + sl; r < r1; r += 4) {
+            const double2 g = G[(int64_t)r * ldg + j], z = Z[(int64_t)r * ldz + j];
+            const double dx = g.x - t * z.x, dy = g.y - t * z.y;
+            s += dx * dx + dy * dy;
+        }
+    }
+    red[sl][l] = s;
+    __syncthreads();
+    if (sl == 0 && j < kc) part[(int64_t)blockIdx.y * kc + j] = (red[0][l] + red[1][l]) + (red[2][l] + red[3][l]);
+}
+
+void launch_resid_cols(hipStream_t st, const double2* G, int ldg, const double2* Z, int ldz, const double* theta,
+                       int n, int kc, int rows_per, double* part) {
+    if (n <= 0 || kc <= 0) return;
+    const int chunks = (n + rows_per - 1) / rows_per;
+    hipLaunchKernelGGL(resid_cols_kernel, dim3((kc + 63) / 64, chunks), dim3(256), 0, st, G, ldg, Z, ldz, theta, n,
+                       kc, rows_per, part);
+    KV_HIP(hipGetLastError());
+}
+
+// ---- overlap over matching block pairs ------------------------------------------------
+
+__global__ __launch_bounds__(256) void dotc_pairs_partial_kernel(const double2* __restrict__ x,
+                                                                 const double2* __restrict__ y,
+                                                                 const DotPair* __restrict__ pairs, int npairs,
+                                                                 double* __restrict__ part) {
+    __shared__ double sr[256], si[256];
+    double re = 0, im = 0;
+    for (int p = 0; p < npairs; ++p) {
+        const DotPair d = pairs[p];
+        for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < d.size; e += (int64_t)gridDim.x * 256) {
+            const double2 a = x[d.x_off + e], b = y[d.y_off + e];
+            re += a.x * b.x + a.y * b.y;  // conj(a) * b
+            im += a.x * b.y - a.y * b.x;
+        }
+    }
+    sr[threadIdx.x] = re;
+    si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        part[2 * blockIdx.x] = sr[0];
+        part[2 * blockIdx.x + 1] = si[0];
+    }
+}
+
+void launch_dotc_pairs(hipStream_t st, const double2* x, const double2* y, const DotPair* pairs, int npairs,
+                       int64_t total, double* part, double2* out) {
+    const int nb = red_blocks(total);  // every block writes its partial, also with no pairs
+    hipLaunchKernelGGL(dotc_pairs_partial_kernel, dim3(nb), dim3(256), 0, st, x, y, pairs, npairs, part);
+    KV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, part, nb, reinterpret_cast<double*>(out), 2);
     KV_HIP(hipGetLastError());
 }
 

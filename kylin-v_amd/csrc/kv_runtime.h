@@ -166,6 +166,17 @@ struct TrdStats {
     long columns = 0;
 };
 
+// The bond SVD's block subspace route: HIP-event time of its GEMM launches with their
+// algorithmic flops (8 m n k) and bytes (operands and result once each, 16 B per complex),
+// of its Cholesky QR kernels (chol_upper + tri_inv) and of its Rayleigh-Ritz eigensolves;
+// sectors solved by the route and sectors redone by the full eigensolver.
+struct BlockStats {
+    double gemm_ms = 0, gemm_flop = 0, gemm_bytes = 0;
+    long gemm_launches = 0;
+    double orth_ms = 0, eig_ms = 0;
+    long orth_calls = 0, eig_calls = 0, sectors = 0, fallbacks = 0;
+};
+
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
@@ -173,6 +184,7 @@ struct Device {
     bool timing = false;  // record HIP events around contraction kernels
     KernelStats stats;
     TrdStats trd;
+    BlockStats blk;
     // GEMM tile autotuner: (precision, opA, opB, shape class) -> tile config
     bool autotune = true;
     std::map<uint64_t, int> gemm_tuned;

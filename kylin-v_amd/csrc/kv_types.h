@@ -64,4 +64,10 @@ struct Copy2D {
     int rows, cols, src_ld, dst_ld, scale_mode, scale_off;
 };
 
+// ---- overlap over matching block pairs ------------------------------------------------
+// sum over pairs of conj(x[x_off + e]) y[y_off + e], e < size
+struct DotPair {
+    int64_t x_off, y_off, size;
+};
+
 }  // namespace kv

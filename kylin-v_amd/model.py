@@ -182,15 +182,32 @@ class HolsteinChain:
     """Device-side MPO, initial MPS and observable MPOs of one chain."""
 
     def __init__(self, N, omegas, gs, Ls, J, layout="merged", ctx=None):
+        self._setup(N, J, layout, chain(N, omegas, gs, Ls, J, layout), ctx)
+
+    @classmethod
+    def from_deck(cls, deck, exciton=0, ctx=None):
+        """The chain of a reference input deck (kln-holstein's -i file, read as
+        holstein/input.h:85-155 reads it), in the projected-purification layout the
+        reference builds, with the exciton in electronic state `exciton` (0-based).  The
+        deck's hopping amplitudes may differ from bond to bond, so hopping() is unavailable."""
+        from . import deck as KD
+        cfg = KD.read_raw(deck)
+        obj = cls.__new__(cls)
+        obj._setup(cfg["num_ele"], None, "pp", KD.lattice_from_deck(cfg, exciton), ctx)
+        obj.deck = cfg
+        return obj
+
+    def _setup(self, N, J, layout, lattice, ctx):
         from . import Tensor, default_context
         self.ctx = ctx or default_context()
         self.N, self.layout = N, layout
-        self.sites, self.terms, self.init, self.ex = chain(N, omegas, gs, Ls, J, layout)
+        self.sites, self.terms, self.init, self.ex = lattice
         self.J = J
         self._T = Tensor
         self.mpo = self._up(mpo_blocks(self.sites, self.terms), (1, 1, -1, -1))
         self.psi0 = self._up(product_state_blocks(self.sites, self.init), (1, 1, -1))
         self._density = {}
+        self._hopping = {}
 
     def _up(self, blist, dirs):
         return [self._T.from_blocks(dirs, b, self.ctx) for b in blist]
@@ -204,9 +221,19 @@ class HolsteinChain:
 
     def hopping(self, i):
         """J (|i><i+1| + h.c.): the bond-energy observable."""
-        a, b = self.ex[i], self.ex[i + 1]
-        terms = [(self.J, [(a, "cdag"), (b, "c")]), (self.J, [(a, "c"), (b, "cdag")])]
-        return self._up(mpo_blocks(self.sites, terms), (1, 1, -1, -1))
+        if self.J is None:
+            raise ValueError("hopping(i) needs a uniform J (a chain built from a deck has none)")
+        if i not in self._hopping:
+            a, b = self.ex[i], self.ex[i + 1]
+            terms = [(self.J, [(a, "cdag"), (b, "c")]), (self.J, [(a, "c"), (b, "cdag")])]
+            self._hopping[i] = self._up(mpo_blocks(self.sites, terms), (1, 1, -1, -1))
+        return self._hopping[i]
+
+    def bond_energies(self, mps):
+        """E_{i,i+1} = <psi| J (|i><i+1| + h.c.) |psi> for every bond (Operator::join over the
+        hopping MPO of each bond, holstein/operator.h:178-198)."""
+        from . import join
+        return np.array([join(self.hopping(i), mps, mps).real for i in range(self.N - 1)])
 
 
 def synthetic_state_blocks(sites, D, seed=0, decay=1.0, Q=1):
@@ -380,3 +407,28 @@ def expanded_state_blocks(sites, init, D, seed=0, eps=0.05, decay=1.0):
             blocks[(qa, qs, qb)] = blk
         out.append(blocks)
     return canonical_blocks(out, maxdim=D)
+
+
+def centre_bond(mps, mpo, mid, Q=1):
+    """Environments and site tensors of a right-canonical device MPS with its centre moved
+    to the bond (mid, mid+1), as a two-site sweep reaches that bond
+    (holstein/tdvp.h:157-175): sites 0..mid-1 left-canonical by exact SVDs
+    (svd_impl with no truncation, holstein/svd.h:296-440), the left environment L_mid
+    built over them, and the right environments over sites mid+2.. and mid+1..
+    Returns (envL, A_mid, A_mid1, envR2, envR1) as device tensors; envR2 is the right
+    environment the two-site H_eff at (mid, mid+1) uses, envR1 the one the backward
+    one-site step at site mid+1 uses."""
+    from . import Tensor, env_sweep, product, svd_impl
+    ctx = mps[0].ctx
+    A = list(mps)
+    envL = Tensor.from_blocks((1, -1, -1), {(0, 0, 0): np.ones((1, 1, 1))}, ctx)
+    for i in range(mid):
+        cur, nex = svd_impl(A[i], 2, -1.0, 10 ** 6, True)
+        A[i] = cur
+        A[i + 1] = product(nex, A[i + 1], 1)
+        envL = env_sweep(envL, A[i], mpo[i], True)
+    envR2 = Tensor.from_blocks((-1, 1, 1), {(Q, 0, Q): np.ones((1, 1, 1))}, ctx)
+    for i in range(len(A) - 1, mid + 1, -1):
+        envR2 = env_sweep(envR2, A[i], mpo[i], False)
+    envR1 = env_sweep(envR2, A[mid + 1], mpo[mid + 1], False)
+    return envL, A[mid], A[mid + 1], envR2, envR1

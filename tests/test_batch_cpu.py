@@ -45,4 +45,4 @@ def test_workers_take_the_costliest_chains_first():
     assert costs == sorted(costs, reverse=True)
     big = max(specs, key=lambda s: s["N"] * s["L"] ** 2 * s["D"])
     assert specs[order[0]] == big
-    assert B.MAX_WORKERS == 16
+    assert B.MAX_WORKERS == 8

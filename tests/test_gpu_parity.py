@@ -426,14 +426,20 @@ def _set_eig(monkeypatch, eig):
     monkeypatch.setenv("KV_TRD", "rocsolver" if eig == "split-roc" else "own")
 
 
-@pytest.mark.parametrize("eig", ["split", "split-roc", "heevd"])
+@pytest.mark.parametrize("eig", ["split", "split-roc", "heevd", "block"])
 def test_svd_gram_route_matches_oracle(kv, eig, monkeypatch):
-    """Sectors whose smaller side exceeds 512 take the Gram route, with the split Hermitian
+    """Sectors whose smaller side is 512, at least twice the block of maxdim + 32, take the
+    block subspace route by default ("block", its Rayleigh-Ritz on the engine's eigensolver);
+    with the block route off they take the full Gram route, with the split Hermitian
     eigensolver (tridiagonalisation, dstedc, back-transformation of the kept vectors) or
-    zheevd: the kept singular values, the truncated product and the isometry match the
+    zheevd.  The kept singular values, the truncated product and the isometry match the
     oracle's dense SVD."""
     print(f"case svd gram eig={eig}", flush=True)
-    _set_eig(monkeypatch, eig)
+    if eig == "block":
+        monkeypatch.delenv("KV_SVD", raising=False)
+    else:
+        _set_eig(monkeypatch, eig)
+        monkeypatch.setenv("KV_SVD", "full")
     from kylin_v_amd import model as KM
     sites, terms, init, ex = KM.chain(6, [1.0], 1.0, [16], 1.0, "merged")
     blocks = KM.synthetic_state_blocks(sites, 64, seed=3, decay=0.3)
@@ -445,7 +451,10 @@ def test_svd_gram_route_matches_oracle(kv, eig, monkeypatch):
     for q in x2.blocks:
         x2.blocks[q] = x2.blocks[q] + 1e-2 * (rng.standard_normal(x2.blocks[q].shape)
                                               + 1j * rng.standard_normal(x2.blocks[q].shape))
-    assert max(min(r, c) for r, c in [(16 * 64, 16 * 64)]) > 512
+    # the sector matrices' sides, from x2's own blocks (rows (a, s1), columns (s2, b))
+    rows, cols = T._sector_layout(x2, 2)
+    sides = [(rows[q][1], cols[-q][1]) for q in rows if -q in cols]
+    assert min(min(r, c) for r, c in sides) >= 512, sides  # Gram routes from 64 (iso_min)
     for maxdim, l2r in [(64, True), (100, False)]:
         lef, rig, sv = T.svd_impl(x2, 2, 1e-10, maxdim, l2r, return_svals=True)
         gl, gr = kv.svd_impl(up(kv, x2), 2, 1e-10, maxdim, l2r)
@@ -533,3 +542,65 @@ def test_two_mode_chain_rank_deficient_gram_sectors(kv):
         c_gpu = kv.overlap_states(ch.psi0, eng.mps)
         assert abs(c_gpu - c_ref) < 1e-10, (k, c_gpu, c_ref)
     assert list(eng.bond_dims()) == list(ref.bond_dims())
+
+
+@pytest.mark.parametrize("eig", ["split", "split-roc"])
+def test_svd_gram_side_beyond_register_rows(kv, eig, monkeypatch):
+    """A Gram sector of side 2112: the tridiagonalisation's column kernel holds 2048 rows of
+    the trailing column in registers (256 threads x 8) and passes the rows beyond them back
+    through the matrix for its first 64 columns.  The full eigensolver route (the block
+    subspace route switched off) on the engine's tridiagonalisation and on rocSOLVER's must
+    both give the oracle's kept values, truncated product and an orthonormal isometry."""
+    print(f"case svd gram side 2112 eig={eig}", flush=True)
+    _set_eig(monkeypatch, eig)
+    monkeypatch.setenv("KV_SVD", "full")
+    rng = np.random.default_rng(21)
+    nr, nc = 2112, 2240
+    sv = np.exp(-np.arange(nr) / 150.0)
+    u, _ = np.linalg.qr(rng.standard_normal((nr, nr)) + 1j * rng.standard_normal((nr, nr)))
+    v, _ = np.linalg.qr(rng.standard_normal((nc, nr)) + 1j * rng.standard_normal((nc, nr)))
+    x = OT((1, -1), {(0, 0): (u * sv[None, :]) @ v.conj().T})
+    for l2r, maxdim in [(True, 200), (False, 260)]:
+        lef, rig = T.svd_impl(x, 1, 1e-10, maxdim, l2r)
+        gl, gr = kv.svd_impl(up(kv, x), 1, 1e-10, maxdim, l2r)
+        gl, gr = down(gl), down(gr)
+        assert gl.legs()[1] == lef.legs()[1]
+        rec_o, rec_g = oproduct(lef, rig, 1), oproduct(gl, gr, 1)
+        assert max_diff(rec_g, rec_o) <= 1e-10 * scale(rec_o)
+        iso = gl if l2r else gr
+        gram = oproduct(iso, iso, 1, "C", "N") if l2r else oproduct(iso, iso, 1, "N", "C")
+        for q, blk in gram.blocks.items():
+            np.testing.assert_allclose(blk, np.eye(blk.shape[0]), atol=1e-12)
+
+
+@pytest.mark.parametrize("l2r", [True, False])
+def test_svd_block_route_cliff_spectrum(kv, l2r):
+    """Block subspace route on sectors of 700 x 900 and 640 x 800 whose values fall from 1
+    to 1e-2 over 60 (50) values and then drop to 1e-7 and below.  Cut at maxdim 100 (maxdim
+    bound, cutoff 1e-10): every sector keeps values above its cliff, the certified values and
+    the ceiling on the rest settle the cut, and the kept values, truncated product and
+    isometry match the oracle.  Cut at maxdim 120 the cut reaches the 1e-7 values, which the
+    block cannot certify: the sectors are redone by the full eigensolver and still match."""
+    print(f"case svd block cliff l2r={l2r}", flush=True)
+    rng = np.random.default_rng(31)
+    t = OT((1, -1))
+
+    def sector(q, nr, nc, top):
+        s = np.concatenate([np.logspace(0, -2, top), 1e-7 * np.logspace(0, -6, 150)])
+        u, _ = np.linalg.qr(rng.standard_normal((nr, len(s))) + 1j * rng.standard_normal((nr, len(s))))
+        v, _ = np.linalg.qr(rng.standard_normal((nc, len(s))) + 1j * rng.standard_normal((nc, len(s))))
+        t.push_back_ch((q, q), (u * s[None, :]) @ v.conj().T)
+
+    sector(0, 700, 900, 60)
+    sector(1, 640, 800, 50)
+    for maxdim in (100, 120):
+        lef, rig = T.svd_impl(t, 1, 1e-10, maxdim, l2r)
+        gl, gr = kv.svd_impl(up(kv, t), 1, 1e-10, maxdim, l2r)
+        gl, gr = down(gl), down(gr)
+        assert gl.legs()[1] == lef.legs()[1], maxdim
+        rec_o, rec_g = oproduct(lef, rig, 1), oproduct(gl, gr, 1)
+        assert max_diff(rec_g, rec_o) <= 1e-10 * scale(rec_o)
+        iso = gl if l2r else gr
+        gram = oproduct(iso, iso, 1, "C", "N") if l2r else oproduct(iso, iso, 1, "N", "C")
+        for q, blk in gram.blocks.items():
+            np.testing.assert_allclose(blk, np.eye(blk.shape[0]), atol=1e-12)

@@ -253,6 +253,40 @@ def test_krylov_stop_rule_and_iteration_cap():
     assert ex2.converged and ex2.err < 1e-8 and ex2.iters < 14
 
 
+def test_krylov_stops_on_the_product_of_residual_norms():
+    """holstein/exponential.h:139-147 multiplies the error estimate by every residual norm
+    |h_k| and stops once the product falls below the tolerance.  On a Hermitian operator whose
+    residual norms stay near 0.1 the product crosses 1e-8 after a few vectors while no single
+    residual norm does before the 14-vector cap, so a last-residual rule would run to the cap.
+    The residual norms come from a textbook three-term Lanczos on the dense operator."""
+    rng = np.random.default_rng(11)
+    n, dt, tol = 40, 0.05, 1e-8
+    q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+    H = (q * np.linspace(-8.0, 8.0, n)[None, :]) @ q.conj().T
+    H = 0.5 * (H + H.conj().T)
+    v = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    # apply_zero with a one-dimensional right leg is y = H c (holstein/product.h:540-545)
+    envl = Tensor((1, -1, -1))
+    envl.push_back_ch((0, 0, 0), H.reshape(n, 1, n))
+    envr = Tensor((-1, 1, 1))
+    envr.push_back_ch((0, 0, 0), np.ones((1, 1, 1), dtype=complex))
+    c = Tensor((1, -1))
+    c.push_back_ch((0, 0), v.reshape(n, 1))
+    A = 0.5 * dt * H
+    vk, vprev, bprev, betas = v / np.linalg.norm(v), np.zeros(n, complex), 0.0, []
+    for _ in range(14):
+        w = A @ vk - bprev * vprev
+        w -= np.vdot(vk, w) * vk
+        betas.append(np.linalg.norm(w))
+        vprev, vk, bprev = vk, w / betas[-1], betas[-1]
+    prod_iters = next(k + 1 for k in range(14) if np.prod(betas[:k + 1]) < tol)
+    last_iters = next((k + 1 for k in range(14) if betas[k] < tol), 14)
+    assert prod_iters < last_iters == 14
+    _, ex = T.local_krylov(envl, c, None, None, envr, dt, tol=tol, max_iter=15, nsite=0)
+    assert ex.converged and ex.iters == prod_iters
+    assert ex.err == pytest.approx(np.prod(betas[:prod_iters]), rel=1e-6)
+
+
 def _svd_fixture(svals_by_sector):
     """Two-leg tensor whose sector q has the given singular values."""
     rng = np.random.default_rng(7)
