@@ -90,6 +90,14 @@ int kv_tensor_download(const kv_tensor* t, double* data);
  * (overlap(a, b), holstein/tensor.h:225-241). */
 int kv_tensor_norm(const kv_tensor* t, double* out);
 int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im);
+/* <b|a> exactly as kv_tensor_overlap computes it (the same kernels, the same order, so the
+ * same bits), allocating no device memory.  It has no counterpart in the reference: it is
+ * for callers that cannot allocate (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The block-pair list, the partial sums and the result
+ * live in a static per-device scratch of 64 KiB (resolved by kv_ctx_create); at most 2048
+ * matching block pairs (KV_ERR_ARG beyond).  Every call on a device uses that device's one
+ * scratch, so calls that overlap in time on one device race. */
+int kv_tensor_overlap_noalloc(const kv_tensor* a, const kv_tensor* b, double* re_im);
 
 /* ---- contractions ---------------------------------------------------------------- */
 /* product<N1,N2,nc>(m1, m2, T1, T2) (holstein/product.h:61-201).  op = 'N', 'T', 'C'.

@@ -74,6 +74,7 @@ SIGNATURES = {
     "kv_tensor_download": (_I, [_P, _DP]),
     "kv_tensor_norm": (_I, [_P, _DP]),
     "kv_tensor_overlap": (_I, [_P, _P, _DP]),
+    "kv_tensor_overlap_noalloc": (_I, [_P, _P, _DP]),
     "kv_product": (_I, [_P, _P, _P, _I, ctypes.c_char, ctypes.c_char, _I, _PP]),
     "kv_heff_profile": (_I, [_P, _I, _P, _P, _P, _P, _P, _I, _I, _DP]),
     "kv_tensor_transpose": (_I, [_P, _P, _IP, _PP]),
@@ -259,6 +260,15 @@ def overlap(ori: Tensor, oth: Tensor) -> complex:
     """<oth|ori> over blocks of equal qn (holstein/tensor.h:225-241)."""
     v = (ctypes.c_double * 2)()
     ori.ctx.check(ori.ctx.lib.kv_tensor_overlap(ori.h, oth.h, v))
+    return complex(v[0], v[1])
+
+
+def overlap_noalloc(ori: Tensor, oth: Tensor) -> complex:
+    """overlap() without any device allocation (kv_tensor_overlap_noalloc): the device's
+    static scratch holds the pair list and partial sums, so calls on one device must not
+    overlap in time."""
+    v = (ctypes.c_double * 2)()
+    ori.ctx.check(ori.ctx.lib.kv_tensor_overlap_noalloc(ori.h, oth.h, v))
     return complex(v[0], v[1])
 
 

@@ -89,6 +89,7 @@ int kv_ctx_create(int device, kv_ctx** out) {
         kv_ctx* c = new kv_ctx;
         c->dev = std::make_shared<kv::Device>(device);
         c->sol.reset(new kv::Solver(*c->dev));
+        kv::noalloc_scratch_init(device);  // the Device constructor made `device` current
         *out = c;
         return KV_OK;
     } catch (const kv::Error& e) {
@@ -223,6 +224,42 @@ int kv_tensor_overlap(const kv_tensor* a, const kv_tensor* b, double* re_im) {
         kv::launch_dotc_pairs(D.stream, B->d(), A->d(), dp.as<kv::DotPair>(), (int)pairs.size(), total,
                               part.as<double>(), out.as<double2>());
         KV_HIP(hipMemcpyAsync(re_im, out.get(), 16, hipMemcpyDeviceToHost, D.stream));
+        D.sync();
+    });
+}
+
+int kv_tensor_overlap_noalloc(const kv_tensor* a, const kv_tensor* b, double* re_im) {
+    if (!a || !b || !re_im) return KV_ERR_ARG;
+    return guard(a->ctx, [&] {
+        kv::Tensor* A = need(a, "a");
+        kv::Tensor* B = need(b, "b");
+        kv::Device& D = *a->ctx->dev;
+        // the pair list is staged on the stack and copied into the device's static scratch
+        // (pairs, then the partial sums, then the result); nothing is allocated
+        kv::DotPair pairs[kv::kNoAllocMaxPairs];
+        int np = 0;
+        int64_t total = 0;
+        bool over = false;
+        for_overlap_pairs(*A, *B, [&](const kv::DotPair& p) {
+            if (np < kv::kNoAllocMaxPairs)
+                pairs[np] = p;
+            else
+                over = true;
+            ++np;
+            total += p.size;
+        });
+        KV_REQUIRE(!over, KV_ERR_ARG,
+                   "kv_tensor_overlap_noalloc: " + std::to_string(np) + " matching blocks exceed the " +
+                       std::to_string(kv::kNoAllocMaxPairs) + " its scratch holds");
+        char* base = static_cast<char*>(kv::noalloc_scratch(D.id));
+        auto* dpairs = reinterpret_cast<kv::DotPair*>(base);
+        auto* part = reinterpret_cast<double*>(base + kv::kNoAllocPairBytes);
+        auto* out = reinterpret_cast<double2*>(base + kv::kNoAllocPairBytes + kv::kNoAllocPartBytes);
+        if (np > 0)
+            KV_HIP(hipMemcpyAsync(dpairs, pairs, (size_t)np * sizeof(kv::DotPair), hipMemcpyHostToDevice,
+                                  D.stream));
+        kv::launch_dotc_pairs(D.stream, B->d(), A->d(), dpairs, np, total, part, out);
+        KV_HIP(hipMemcpyAsync(re_im, out, 16, hipMemcpyDeviceToHost, D.stream));
         D.sync();
     });
 }

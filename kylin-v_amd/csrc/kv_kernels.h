@@ -115,4 +115,14 @@ constexpr int kRedBlocks = 1024;
 void launch_dotc_pairs(hipStream_t st, const double2* x, const double2* y, const DotPair* pairs, int npairs,
                        int64_t total, double* part, double2* out);
 
+// Per-device static scratch of the allocation-free overlap entry point: the pair list
+// (kNoAllocMaxPairs DotPair), the partial sums and the result.  noalloc_scratch_init resolves
+// its address on the current device (once per device); noalloc_scratch returns it.
+constexpr int kNoAllocMaxPairs = 2048;
+constexpr size_t kNoAllocPairBytes = (size_t)kNoAllocMaxPairs * sizeof(DotPair);
+constexpr size_t kNoAllocPartBytes = (size_t)2 * kRedBlocks * sizeof(double);
+constexpr size_t kNoAllocScratchBytes = kNoAllocPairBytes + kNoAllocPartBytes + 256;
+void noalloc_scratch_init(int device);
+void* noalloc_scratch(int device);
+
 }  // namespace kv

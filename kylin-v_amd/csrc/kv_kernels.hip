@@ -11,6 +11,8 @@
 //  * wfold: applies the sparse MPO tensor as a gather-sum over contiguous slices.
 //  * perm / copy2d: block transposes and sector gather/scatter.
 //  * FP64 vector algebra for the Lanczos recurrence (deterministic two-pass reductions).
+#include <mutex>
+
 #include "kv_kernels.h"
 #include "kv_runtime.h"
 
@@ -1680,7 +1682,7 @@ void launch_resid_cols(hipStream_t st, const double2* G, int ldg, const double2*
     KV_HIP(hipGetLastError());
 }
 
-// ---- overlap over matching block pairs ------------------------------------------------
+// ---- overlap over matching block pairs, and the allocation-free entry point's scratch -----
 
 __global__ __launch_bounds__(256) void dotc_pairs_partial_kernel(const double2* __restrict__ x,
                                                                  const double2* __restrict__ y,
@@ -1719,6 +1721,40 @@ void launch_dotc_pairs(hipStream_t st, const double2* x, const double2* y, const
     KV_HIP(hipGetLastError());
     hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, part, nb, reinterpret_cast<double*>(out), 2);
     KV_HIP(hipGetLastError());
+}
+
+// The allocation-free entry point's scratch.  In host code `scratch` names a host-side
+// stand-in, not device memory: only the address hipGetSymbolAddress returns for the
+// current device may be handed to kernels and copies.
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex& scratch_mutex() {
+    static std::mutex m;
+    return m;
+}
+void* scratch_ptr[kMaxDevices] = {nullptr};
+}  // namespace
+
+void noalloc_scratch_init(int device) {
+    KV_REQUIRE(device >= 0 && device < kMaxDevices, ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> g(scratch_mutex());
+    if (scratch_ptr[device]) return;
+    int cur = -1;
+    KV_HIP(hipGetDevice(&cur));
+    KV_REQUIRE(cur == device, ERR_INTERNAL, "noalloc scratch: device is not current");
+    void* p = nullptr;
+    KV_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    KV_REQUIRE(p != nullptr, ERR_HIP, "hipGetSymbolAddress returned no address for the scratch");
+    scratch_ptr[device] = p;
+}
+
+void* noalloc_scratch(int device) {
+    KV_REQUIRE(device >= 0 && device < kMaxDevices, ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> g(scratch_mutex());
+    KV_REQUIRE(scratch_ptr[device] != nullptr, ERR_INTERNAL, "noalloc scratch not initialised on this device");
+    return scratch_ptr[device];
 }
 
 }  // namespace kv
