@@ -75,6 +75,7 @@ struct SvdResult {
     std::vector<std::vector<double>> svals;  // per sector pair, kept values
     int gram_sectors = 0, exact_sectors = 0, refined_sectors = 0;
     int block_sectors = 0, block_fallbacks = 0;  // block subspace route: kept, redone in full
+    std::map<int, int> kept_by_q;                 // kept dimension per row-sector charge
     std::string detail;  // per sector: rows x cols, route, s_max, last kept value
 };
 
@@ -93,8 +94,10 @@ struct SvdOptions {
     bool force_exact = false;
     int eig_mode = EIG_AUTO;  // Gram eigensolver: split from kSplitEigMin, or forced
     bool roc_trd = false;     // split route: rocSOLVER zhetrd instead of hetrd_lower
+    bool qr_reduce = true;    // Householder-reduce long isometric sides before their Gram route
     static SvdOptions from_env() {
         SvdOptions o;
+        if (const char* e = std::getenv("KV_QR_REDUCE")) o.qr_reduce = std::string(e) != "0";
         if (const char* e = std::getenv("KV_TRD")) o.roc_trd = std::string(e) == "rocsolver";
         if (const char* e = std::getenv("KV_SVD")) {
             o.force_exact = std::string(e) == "exact";
@@ -160,6 +163,11 @@ struct SectorSvd {
     DevBuf Z, W;
     int kc = 0;
     double ceil = 0.0;  // every singular value beyond the certified ones is at most this
+    // Householder reduction (qr_reduce): M holds the square triangular factor and Qf the
+    // orthonormal one, row-major; the isometric side was `full` long
+    bool reduced = false;
+    int64_t full = 0;
+    DevBuf Qf;
 };
 
 inline void exact_sector(Device& D, Solver& sol, SectorSvd& X, bool keep_zero) {
@@ -228,6 +236,63 @@ inline void gram_prepare(Device& D, SectorSvd& X, bool iso = false, bool colside
     X.gscale = f2 > 0 ? f2 : 1.0;
     launch_zaxpby(D.stream, X.G.as<double2>(), X.G.as<double2>(), nullptr,
                   make_double2(1.0 / X.gscale, 0.0), make_double2(0.0, 0.0), (int64_t)n * n);
+}
+
+// Householder reduction of a sector whose isometric side is the longer one.  The row-major
+// buffer of M (nr x nc) is the column-major nc x nr matrix M^T.  Column-side isometry
+// (nc > nr): M^T = Q R (zgeqrf), so M = R^T Q^T and the sector continues as the nr x nr
+// lower-triangular R^T, with V^H = V'^H Q^T at the end.  Row-side isometry (nr > nc):
+// M^T = L Q (zgelqf), M = Q^T L^T, the sector continues as the nc x nc upper-triangular L^T,
+// with U = Q^T U'.  Both triangles sit where the row-major M had them (element (i, j) at
+// i * nc + j), and the explicit Q (zungqr / zunglq) read row-major is Q^T.  The isometric
+// Gram matrix is then square in the shorter side instead of the longer one.
+inline void qr_reduce(Device& D, Solver& sol, SectorSvd& X, bool colside) {
+    const int r = (int)X.nr, c = (int)X.nc;
+    const int m = colside ? r : c;
+    DevBuf A(D.pool.get(), X.M.bytes()), tau(D.pool.get(), (size_t)m * sizeof(double2));
+    KV_HIP(hipMemcpyAsync(A.get(), X.M.get(), X.M.bytes(), hipMemcpyDeviceToDevice, D.stream));
+    auto* a = reinterpret_cast<rocblas_double_complex*>(A.get());
+    auto* t = reinterpret_cast<rocblas_double_complex*>(tau.get());
+    auto ok = [](rocblas_status st, const char* what) {
+        KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, std::string(what) + " failed");
+    };
+    if (colside)
+        ok(rocsolver_zgeqrf(sol.h, c, r, a, c, t), "rocsolver_zgeqrf");
+    else
+        ok(rocsolver_zgelqf(sol.h, c, r, a, c, t), "rocsolver_zgelqf");
+    DevBuf T(D.pool.get(), (size_t)m * m * sizeof(double2));
+    launch_tri_copy(D.stream, A.as<double2>(), c, T.as<double2>(), m, colside);
+    if (colside)
+        ok(rocsolver_zungqr(sol.h, c, r, r, a, c, t), "rocsolver_zungqr");
+    else
+        ok(rocsolver_zunglq(sol.h, c, r, c, a, c, t), "rocsolver_zunglq");
+    D.sync();
+    X.reduced = true;
+    X.full = colside ? c : r;
+    X.Qf = std::move(A);
+    X.M = std::move(T);
+    X.nr = X.nc = m;
+}
+
+// Undo qr_reduce on the formed factors: V^H (k x m) <- V^H Q^T (k x nc), or U (m x ldU)
+// <- Q^T U (nr x ldU).
+inline void qr_expand(Device& D, SectorSvd& X, bool colside, int nk) {
+    const int m = (int)X.nr;
+    const int n = (int)X.full;
+    if (colside) {
+        DevBuf Vh(D.pool.get(), (size_t)nk * n * sizeof(double2));
+        gemm_dense(D, 'N', 'N', X.Vh.as<double2>(), m, X.Qf.as<double2>(), n, Vh.as<double2>(), n, nk, n, m);
+        X.Vh = std::move(Vh);
+        X.nc = n;
+    } else {
+        DevBuf U(D.pool.get(), (size_t)n * X.ldU * sizeof(double2));
+        gemm_dense(D, 'N', 'N', X.Qf.as<double2>(), m, X.U.as<double2>(), X.ldU, U.as<double2>(), X.ldU, n,
+                   X.ldU, m);
+        X.U = std::move(U);
+        X.nr = n;
+    }
+    X.Qf = DevBuf();
+    X.reduced = false;
 }
 
 // Reflectors per block of the back-transformation.
@@ -731,275 +796,388 @@ inline void iso_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
 // ---- block subspace route -------------------------------------------------------------
 // A two-site sector matrix of a D = 256 bond has sides in the thousands (2048 x 4096 for the
 // 32-site chain) but keeps at most maxdim values.  Its leading singular subspace on the
-// isometric side comes from a seeded random start block of maxdim + kBlockOversample
-// columns with one power step, orthonormalised between the products (randomised range
-// finder with subspace iteration: Halko, Martinsson and Tropp, SIAM Review 53 (2011),
-// algorithms 4.3/4.4), then Rayleigh-Ritz on the block.  Each Ritz pair's residual
-// |G z - theta z| (G the Gram matrix of the isometric side) is measured, and only the leading
-// pairs whose residual is at most kBlockResid x the largest Ritz value count as singular
-// values.  A sector whose truncation would keep every such value, or reach below the tail
-// threshold where the Gram eigenvalues lose accuracy, is redone by the full eigensolver.
+// isometric side comes from a seeded random start block of b columns with power steps,
+// orthonormalised by Cholesky QR between the products (randomised range finder with
+// subspace iteration: Halko, Martinsson and Tropp, SIAM Review 53 (2011), algorithms
+// 4.3/4.4), then Rayleigh-Ritz on the block.  b is maxdim plus kBlockOversample, trimmed to
+// 256 when that still holds maxdim.  Each Ritz pair's residual |G z - theta z| (G the Gram matrix of the
+// isometric side) is measured; the leading pairs within kBlockResid x the largest Ritz value
+// count as singular values, and |M|_F^2 minus their sum bounds every other value of the
+// sector.  A cut that keeps only certified values above the tail threshold and leaves the
+// bound below it (below the cutoff, or below the smallest kept value when maxdim binds) is
+// settled; otherwise the sector takes another power step, and after kBlockPowerMax of them
+// the full eigensolver.  All sectors of a bond in the route run together: one grouped GEMM
+// launch per product, one batched Cholesky QR, one batched Rayleigh-Ritz eigensolve.
 constexpr int kBlockOversample = 32;
+constexpr int kQrReduceRatio = 2;  // isometric side / other side from which qr_reduce runs
+constexpr int kBlockEigSmall = 256;    // largest side of rocSOLVER's one-kernel tridiagonalisation
 constexpr double kBlockDrop = 1e-12;   // Cholesky QR pivot threshold (x the largest)
 constexpr double kBlockResid = 1e-13;  // Ritz residual bound (x the largest Ritz value)
 constexpr int kBlockPowerMax = 3;      // power steps before the sector goes to the full route
 // rounding allowance of |M|_F^2 - sum of certified theta (relative to |M|_F^2)
 constexpr double kBlockMassSlack = 1e-13;
 
-// C (m x n, ld n) = op(A) op(B).  When the output has too few 64 x 64 tiles to fill the GPU,
-// K is cut into chunks whose partial products a second kernel sums in chunk order.
-inline void gemm_chunked(Device& D, char opA, char opB, const double2* A, int lda, const double2* B, int ldb,
-                         double2* C, int m, int n, int k) {
-    const int tiles = ((m + GEMM_BM - 1) / GEMM_BM) * ((n + GEMM_BN - 1) / GEMM_BN);
-    int nch = 1;
-    while (tiles * nch < 512 && k / (2 * nch) >= 256) nch *= 2;
-    EventTimer ev;
-    ev.start(D.stream);
-    D.blk.gemm_flop += 8.0 * m * n * (double)k;
-    D.blk.gemm_bytes += 16.0 * ((double)m * k + (double)k * n + (double)m * n);
-    D.blk.gemm_launches += 1;
-    if (nch == 1) {
-        gemm_dense(D, opA, opB, A, lda, B, ldb, C, n, m, n, k);
-        ev.stop(D.stream);
-        D.blk.gemm_ms += ev.ms();
-        return;
+// One dense product of a batched launch: C (m x n, ld n) = op(A) op(B).
+struct GemmJob {
+    const double2* A;
+    int lda;
+    const double2* B;
+    int ldb;
+    double2* C;
+    int m, n, k;
+};
+
+// Every job's product in one grouped GEMM launch (the jobs share opA and opB).  Operand
+// offsets are measured from the lowest address among the jobs' operands, so their buffers may
+// be separate allocations.  When the outputs together have too few 64 x 64 tiles to fill the
+// GPU, K is cut into chunks whose partial products a second kernel sums in chunk order.
+inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>& jobs) {
+    if (jobs.empty()) return;
+    int tiles = 0, kmin = jobs[0].k;
+    const double2 *a0 = jobs[0].A, *b0 = jobs[0].B;
+    double2* cmin = jobs[0].C;
+    for (const auto& j : jobs) {
+        KV_REQUIRE(j.k > 0 && j.m > 0 && j.n > 0, ERR_INTERNAL, "block route: empty product");
+        tiles += ((j.m + GEMM_BM - 1) / GEMM_BM) * ((j.n + GEMM_BN - 1) / GEMM_BN);
+        kmin = std::min(kmin, j.k);
+        a0 = std::min(a0, j.A);
+        b0 = std::min(b0, j.B);
+        cmin = std::min(cmin, j.C);
     }
-    const int kc = ((k + nch - 1) / nch + 7) / 8 * 8;
-    nch = (k + kc - 1) / kc;
-    DevBuf part(D.pool.get(), (size_t)nch * m * n * sizeof(double2));
+    int nch = 1;
+    while (tiles * nch < 512 && kmin / (2 * nch) >= 256) nch *= 2;
+    size_t part_elems = 0;
+    if (nch > 1)
+        for (const auto& j : jobs) part_elems += (size_t)nch * j.m * j.n;
+    DevBuf part(D.pool.get(), part_elems * sizeof(double2));
+    double2* c0 = nch > 1 ? part.as<double2>() : cmin;
     std::vector<GemmGroup> gs;
     std::vector<GemmSeg> ss;
     std::vector<GemmTile> ts;
-    for (int c = 0; c < nch; ++c) {
-        const int k0 = c * kc;
-        GemmGroup g{};
-        g.c_off = (int64_t)c * m * n;
-        g.m = m;
-        g.n = n;
-        g.ldc = n;
-        g.seg_begin = c;
-        g.seg_count = 1;
-        GemmSeg sg{};
-        sg.a_off = opA == 'N' ? (int64_t)k0 : (int64_t)k0 * lda;
-        sg.b_off = opB == 'N' ? (int64_t)k0 * ldb : (int64_t)k0;
-        sg.k = std::min(kc, k - k0);
-        sg.lda = lda;
-        sg.ldb = ldb;
-        gs.push_back(g);
-        ss.push_back(sg);
-        for (int a = 0; a < (m + GEMM_BM - 1) / GEMM_BM; ++a)
-            for (int b = 0; b < (n + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{c, a, b, 0});
+    std::vector<int64_t> poff(jobs.size());
+    int64_t pos = 0;
+    double flop = 0, bytes = 0;
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        const GemmJob& j = jobs[q];
+        // chunks of at least 512 deep (kmin >= 512 nch), so none is empty after rounding
+        const int kc = nch == 1 ? j.k : ((j.k + nch - 1) / nch + 7) / 8 * 8;
+        poff[q] = pos;
+        for (int c = 0; c < nch; ++c) {
+            const int k0 = c * kc;
+            GemmGroup g{};
+            g.c_off = nch > 1 ? pos + (int64_t)c * j.m * j.n : (int64_t)(j.C - c0);
+            g.m = j.m;
+            g.n = j.n;
+            g.ldc = j.n;
+            g.seg_begin = (int)ss.size();
+            g.seg_count = 1;
+            GemmSeg sg{};
+            sg.a_off = (int64_t)(j.A - a0) + (opA == 'N' ? (int64_t)k0 : (int64_t)k0 * j.lda);
+            sg.b_off = (int64_t)(j.B - b0) + (opB == 'N' ? (int64_t)k0 * j.ldb : (int64_t)k0);
+            sg.k = std::min(kc, j.k - k0);
+            sg.lda = j.lda;
+            sg.ldb = j.ldb;
+            KV_REQUIRE(sg.k > 0, ERR_INTERNAL, "block route: empty K chunk");
+            const int gi = (int)gs.size();
+            gs.push_back(g);
+            ss.push_back(sg);
+            for (int a = 0; a < (j.m + GEMM_BM - 1) / GEMM_BM; ++a)
+                for (int b = 0; b < (j.n + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{gi, a, b, 0});
+        }
+        if (nch > 1) pos += (int64_t)nch * j.m * j.n;
+        flop += 8.0 * j.m * j.n * (double)j.k;
+        bytes += 16.0 * ((double)j.m * j.k + (double)j.k * j.n + (double)j.m * j.n);
     }
     DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
-    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, part.as<double2>(), dt.as<GemmTile>(),
-                         (int)ts.size(), dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
-    launch_sum_parts(D.stream, part.as<double2>(), nch, (int64_t)m * n, C);
+    EventTimer ev;
+    ev.start(D.stream);
+    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), a0, b0, c0, dt.as<GemmTile>(), (int)ts.size(),
+                         dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
+    if (nch > 1)
+        for (size_t q = 0; q < jobs.size(); ++q)
+            launch_sum_parts(D.stream, part.as<double2>() + poff[q], nch, (int64_t)jobs[q].m * jobs[q].n,
+                             jobs[q].C);
     ev.stop(D.stream);
     D.blk.gemm_ms += ev.ms();
+    D.blk.gemm_flop += flop;
+    D.blk.gemm_bytes += bytes;
+    D.blk.gemm_launches += 1;
     D.sync();
 }
 
-// Orthonormal basis of the columns of Y (n x c, row-major, ld c) by two Cholesky QR passes,
-// Q = Y R^{-1} with R from Y^H Y.  Columns whose pivot is at most kBlockDrop x the largest
-// are dependent on the earlier ones and are dropped, so the basis has r <= c columns
-// (returned, n x r, ld r).  One pass leaves an orthogonality error ~eps cond(Y)^2 <= 1e-4
-// on the kept columns; the second brings it to rounding.
-inline DevBuf block_orth(Device& D, DevBuf Y, int n, int c, int& r) {
-    int cc = c;
-    for (int pass = 0; pass < 2 && cc > 0; ++pass) {
-        KV_REQUIRE(cc <= kCholMax, ERR_INTERNAL, "block route: block wider than the Cholesky kernel");
-        const size_t cc2 = (size_t)cc * cc;
-        DevBuf S(D.pool.get(), cc2 * sizeof(double2)), T(D.pool.get(), cc2 * sizeof(double2));
-        DevBuf live(D.pool.get(), (size_t)cc * sizeof(int));
-        DevBuf sz = upload_vec(D, std::vector<int>{cc});
-        gemm_chunked(D, 'C', 'N', Y.as<double2>(), cc, Y.as<double2>(), cc, S.as<double2>(), cc, cc, n);
-        EventTimer ev;
-        ev.start(D.stream);
-        launch_chol_upper(D.stream, S.as<double2>(), sz.as<int>(), cc, (int64_t)cc2, kBlockDrop, live.as<int>(), 1);
-        launch_tri_inv(D.stream, S.as<double2>(), T.as<double2>(), sz.as<int>(), live.as<int>(), cc, (int64_t)cc2,
-                       cc, 1);
-        ev.stop(D.stream);
-        D.blk.orth_ms += ev.ms();
-        D.blk.orth_calls += 1;
-        std::vector<int> hl(cc);
-        KV_HIP(hipMemcpyAsync(hl.data(), live.get(), (size_t)cc * sizeof(int), hipMemcpyDeviceToHost, D.stream));
-        DevBuf Q(D.pool.get(), (size_t)n * cc * sizeof(double2));
-        gemm_chunked(D, 'N', 'N', Y.as<double2>(), cc, T.as<double2>(), cc, Q.as<double2>(), n, cc, cc);
-        int nl = 0;
-        for (int v : hl) nl += v ? 1 : 0;
-        if (nl < cc && nl > 0) {  // keep the live columns, in order, as runs of adjacent columns
-            std::vector<Copy2D> cps;
-            int dst = 0;
-            for (int j = 0; j < cc;) {
-                if (!hl[j]) {
-                    ++j;
-                    continue;
-                }
-                int e = j;
-                while (e < cc && hl[e]) ++e;
-                Copy2D cp{};
-                cp.src_off = j;
-                cp.dst_off = dst;
-                cp.rows = n;
-                cp.cols = e - j;
-                cp.src_ld = cc;
-                cp.dst_ld = nl;
-                cps.push_back(cp);
-                dst += e - j;
-                j = e;
-            }
-            DevBuf Qc(D.pool.get(), (size_t)n * nl * sizeof(double2));
-            DevBuf dc = upload_vec(D, cps);
-            launch_copy2d<double2>(D.stream, Q.as<double2>(), Qc.as<double2>(), dc.as<Copy2D>(), (int)cps.size(), n,
-                                   nullptr);
-            D.sync();
-            Q = std::move(Qc);
-        }
-        Y = std::move(Q);
-        cc = nl;
+// One sector of a bond in the block route.  A is M (isometry on the rows) or M^H (on the
+// columns), so that A's rows are the isometric side: ni x no.
+struct BlockJob {
+    SectorSvd* X = nullptr;
+    int ni = 0, no = 0;
+    uint64_t seed = 0;
+    double f2 = 0;             // |M|_F^2
+    DevBuf Q, P;               // isometric-side basis (ni x b) and A^H Q (no x b)
+    int power = 0;             // power steps taken
+    int cert = 0;              // certified Ritz pairs at the last Rayleigh-Ritz
+    const char* fallback = ""; // why the sector left the route (trace output)
+    std::vector<double> theta; // leading Ritz values, descending
+};
+
+// out_j = A_j in_j (herm false: ni x c) or A_j^H in_j (herm true: no x c) for every job.
+inline void block_amul(Device& D, std::vector<BlockJob*>& J, bool iso_rows, bool herm,
+                       const std::vector<const double2*>& in, int c, const std::vector<double2*>& out) {
+    const bool conj_m = iso_rows == herm;  // the product takes M^H
+    std::vector<GemmJob> g;
+    for (size_t q = 0; q < J.size(); ++q) {
+        const SectorSvd& X = *J[q]->X;
+        const int nr = (int)X.nr, nc = (int)X.nc;
+        if (conj_m)
+            g.push_back(GemmJob{X.M.as<double2>(), nc, in[q], c, out[q], nc, c, nr});
+        else
+            g.push_back(GemmJob{X.M.as<double2>(), nc, in[q], c, out[q], nr, c, nc});
     }
-    r = cc;
-    return Y;
+    gemm_jobs(D, conj_m ? 'C' : 'N', 'N', g);
 }
 
-// The block route for one sector with the isometry on its rows (iso_rows, S going right) or
-// its columns: A = M or M^H so that A's rows are the isometric side (ni x no).
-//   Y = A Omega (Omega no x b seeded uniform),  Q = orth(Y),  then per power step
-//   P = orth(A^H Q),  Q = orth(A P),  P = A^H Q,  H = P^H P = Q^H A A^H Q = V Theta V^H,
-//   Z = Q V (Ritz vectors),  W = A^H Z = P V,  residual_j = |A W_j - theta_j Z_j|,
-// repeated (at most kBlockPowerMax steps) until the leading min(r, kmax) pairs all pass.
-// X.s receives the leading singular values sqrt(theta_j) whose residuals pass, X.Z and X.W
-// the leading Ritz vectors and their products, X.ceil the bound on every other value.
-// Returns false if no pair passes.
-inline bool block_sector(Device& D, Solver& sol, SectorSvd& X, bool iso_rows, int b, int kmax, uint64_t seed,
-                         int mode, bool roc_trd) {
-    const int nr = (int)X.nr, nc = (int)X.nc;
-    const int ni = iso_rows ? nr : nc, no = iso_rows ? nc : nr;
-    const double2* m = X.M.as<double2>();
-    auto amul = [&](const double2* Bm, int c, double2* out) {  // out (ni x c) = A Bm
-        if (iso_rows)
-            gemm_chunked(D, 'N', 'N', m, nc, Bm, c, out, nr, c, nc);
-        else
-            gemm_chunked(D, 'C', 'N', m, nc, Bm, c, out, nc, c, nr);
-    };
-    auto amul_h = [&](const double2* Qm, int c, double2* out) {  // out (no x c) = A^H Qm
-        if (iso_rows)
-            gemm_chunked(D, 'C', 'N', m, nc, Qm, c, out, nc, c, nr);
-        else
-            gemm_chunked(D, 'N', 'N', m, nc, Qm, c, out, nr, c, nc);
-    };
-    DevBuf Y(D.pool.get(), (size_t)ni * b * sizeof(double2));
-    {
-        DevBuf Om(D.pool.get(), (size_t)no * b * sizeof(double2));
-        launch_rand_fill(D.stream, Om.as<double2>(), (int64_t)no * b, seed);
-        amul(Om.as<double2>(), b, Y.as<double2>());
-    }
-    int r = 0;
-    DevBuf Q = block_orth(D, std::move(Y), ni, b, r);
-    if (r == 0) return false;
-    double f2 = 0.0;
-    {
-        DevBuf sc(D.pool.get(), 1024 * sizeof(double)), nr2(D.pool.get(), sizeof(double));
-        launch_nrm2sq(D.stream, m, X.nr * X.nc, nr2.as<double>(), sc.as<double>());
-        KV_HIP(hipMemcpyAsync(&f2, nr2.get(), sizeof(double), hipMemcpyDeviceToHost, D.stream));
-        D.sync();
-    }
-    const double gscale = f2 > 0 ? f2 : 1.0;
-    DevBuf P;
-    std::vector<double> theta;
-    int kc = 0, cert = 0;
-    for (int q = 1; q <= kBlockPowerMax; ++q) {
-        // one power step, orthonormalised on both sides: Q <- orth(A orth(A^H Q))
-        P = DevBuf(D.pool.get(), (size_t)no * r * sizeof(double2));
-        amul_h(Q.as<double2>(), r, P.as<double2>());
-        int rp = 0;
-        P = block_orth(D, std::move(P), no, r, rp);
-        if (rp == 0) return false;
-        Q = DevBuf(D.pool.get(), (size_t)ni * rp * sizeof(double2));
-        amul(P.as<double2>(), rp, Q.as<double2>());
-        Q = block_orth(D, std::move(Q), ni, rp, r);
-        if (r == 0) return false;
-        P = DevBuf(D.pool.get(), (size_t)no * r * sizeof(double2));
-        amul_h(Q.as<double2>(), r, P.as<double2>());
-        // Rayleigh-Ritz at unit scale (the eigensolver sees the same scale as in the full route)
-        SectorSvd H;
-        H.k = r;
-        H.gram = true;
-        H.G = DevBuf(D.pool.get(), (size_t)r * r * sizeof(double2));
-        gemm_chunked(D, 'C', 'N', P.as<double2>(), r, P.as<double2>(), r, H.G.as<double2>(), r, r, no);
-        launch_zaxpby(D.stream, H.G.as<double2>(), H.G.as<double2>(), nullptr, make_double2(1.0 / gscale, 0.0),
-                      make_double2(0.0, 0.0), (int64_t)r * r);
-        {
+// Orthonormal bases of the columns of every Y_j (n_j x b, row-major), in place: Cholesky QR
+// passes (one leaves an orthogonality error ~eps cond(Y)^2, a second brings it to rounding).
+// Columns whose pivot falls to kBlockDrop x the largest depend on the earlier ones: with
+// `refill` they are replaced by seeded random vectors at the scale of the largest column and
+// the pass repeats (at most twice), so the basis keeps b independent columns; without, they
+// come out zero and stay zero through the products that follow.
+inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int>& n, int b, int passes,
+                       bool refill, uint64_t seed) {
+    const int nj = (int)Y.size();
+    KV_REQUIRE(b <= kCholMax, ERR_INTERNAL, "block route: block wider than the Cholesky kernel");
+    // blocks up to kCholBlockedMax: panel-blocked Cholesky and a row-parallel triangular solve;
+    // wider: the unblocked factor, its inverse and a GEMM
+    const bool blocked = b <= kCholBlockedMax;
+    const size_t bb = (size_t)b * b;
+    const int64_t rds = (int64_t)((b + kCholNB - 1) / kCholNB) * kCholNB * kCholNB;
+    DevBuf S(D.pool.get(), nj * bb * sizeof(double2));
+    DevBuf T(D.pool.get(), blocked ? 0 : nj * bb * sizeof(double2));
+    DevBuf Rd(D.pool.get(), blocked ? (size_t)nj * rds * sizeof(double2) : 0);
+    DevBuf live(D.pool.get(), (size_t)nj * b * sizeof(int)), dmax(D.pool.get(), (size_t)nj * sizeof(double));
+    DevBuf sz = upload_vec(D, std::vector<int>(nj, b));
+    std::vector<int> hl((size_t)nj * b);
+    for (int pass = 0; pass < passes; ++pass) {
+        for (int attempt = 0;; ++attempt) {
+            std::vector<GemmJob> g;
+            for (int q = 0; q < nj; ++q)
+                g.push_back(GemmJob{Y[q]->as<double2>(), b, Y[q]->as<double2>(), b, S.as<double2>() + q * bb, b, b,
+                                    n[q]});
+            gemm_jobs(D, 'C', 'N', g);
             EventTimer ev;
             ev.start(D.stream);
-            gram_solve(D, sol, {&H}, mode, roc_trd);
-            gram_eigvecs(D, sol, H, 0);
-            ev.stop(D.stream);
-            D.blk.eig_ms += ev.ms();
-            D.blk.eig_calls += 1;
-        }
-        kc = std::min(r, kmax);
-        DevBuf Vd(D.pool.get(), (size_t)kc * r * sizeof(double2));  // rows v_j^H, descending
-        {
-            std::vector<Copy2D> cps(kc);
-            for (int j = 0; j < kc; ++j) {
-                Copy2D c{};
-                c.src_off = (int64_t)(r - 1 - j - H.ev_lo) * r;
-                c.dst_off = (int64_t)j * r;
-                c.rows = 1;
-                c.cols = r;
-                c.src_ld = r;
-                c.dst_ld = r;
-                cps[j] = c;
+            if (blocked) {
+                launch_chol_blocked(D.stream, S.as<double2>(), sz.as<int>(), b, (int64_t)bb, kBlockDrop,
+                                    live.as<int>(), dmax.as<double>(), Rd.as<double2>(), rds, nj);
+            } else {
+                launch_chol_upper(D.stream, S.as<double2>(), sz.as<int>(), b, (int64_t)bb, kBlockDrop, live.as<int>(),
+                                  dmax.as<double>(), nj);
+                launch_tri_inv(D.stream, S.as<double2>(), T.as<double2>(), sz.as<int>(), live.as<int>(), b,
+                               (int64_t)bb, b, nj);
             }
-            DevBuf dc = upload_vec(D, cps);
-            launch_copy2d<double2>(D.stream, H.Ev.as<double2>(), Vd.as<double2>(), dc.as<Copy2D>(), kc, 1, nullptr);
+            ev.stop(D.stream);
+            D.blk.orth_ms += ev.ms();
+            D.blk.orth_calls += 1;
+            KV_HIP(hipMemcpyAsync(hl.data(), live.get(), hl.size() * sizeof(int), hipMemcpyDeviceToHost, D.stream));
+            D.sync();
+            bool dead = false;
+            for (int v : hl) dead = dead || v == 0;
+            if (!dead || !refill || attempt == 2) break;
+            for (int q = 0; q < nj; ++q)
+                launch_refill_cols(D.stream, Y[q]->as<double2>(), n[q], b, live.as<int>() + (size_t)q * b,
+                                   dmax.as<double>() + q, seed + 977u * (uint64_t)(pass * 3 + attempt + 1) + q);
         }
-        X.Z = DevBuf(D.pool.get(), (size_t)ni * kc * sizeof(double2));
-        X.W = DevBuf(D.pool.get(), (size_t)no * kc * sizeof(double2));
-        gemm_chunked(D, 'N', 'C', Q.as<double2>(), r, Vd.as<double2>(), r, X.Z.as<double2>(), ni, kc, r);
-        gemm_chunked(D, 'N', 'C', P.as<double2>(), r, Vd.as<double2>(), r, X.W.as<double2>(), no, kc, r);
-        // residuals of the Ritz pairs: A W_j = G z_j against theta_j z_j
-        theta.assign(kc, 0.0);
-        for (int j = 0; j < kc; ++j) theta[j] = H.s[j] * H.s[j] * gscale;
-        DevBuf GZ(D.pool.get(), (size_t)ni * kc * sizeof(double2));
-        amul(X.W.as<double2>(), kc, GZ.as<double2>());
-        constexpr int rows_per = 256;
-        const int chunks = (ni + rows_per - 1) / rows_per;
-        DevBuf dth = upload_vec(D, theta);
-        DevBuf part(D.pool.get(), (size_t)chunks * kc * sizeof(double));
-        launch_resid_cols(D.stream, GZ.as<double2>(), kc, X.Z.as<double2>(), kc, dth.as<double>(), ni, kc,
-                          rows_per, part.as<double>());
-        std::vector<double> hp((size_t)chunks * kc);
-        KV_HIP(hipMemcpyAsync(hp.data(), part.get(), hp.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+        for (int q = 0; q < nj; ++q) {
+            DevBuf Qn(D.pool.get(), (size_t)n[q] * b * sizeof(double2));
+            if (blocked) {
+                EventTimer ev;
+                ev.start(D.stream);
+                launch_trsm_rows(D.stream, Y[q]->as<double2>(), Qn.as<double2>(), n[q], b, S.as<double2>() + q * bb, b,
+                                 Rd.as<double2>() + (size_t)q * rds);
+                ev.stop(D.stream);
+                D.blk.orth_ms += ev.ms();
+            } else {
+                std::vector<GemmJob> g{GemmJob{Y[q]->as<double2>(), b, T.as<double2>() + q * bb, b, Qn.as<double2>(),
+                                               n[q], b, b}};
+                gemm_jobs(D, 'N', 'N', g);
+            }
+            *Y[q] = std::move(Qn);
+        }
+    }
+}
+
+// Start blocks: Q_j = orth(A_j Omega_j) with Omega_j (no x b) seeded uniform, one pass.
+inline void block_start(Device& D, std::vector<BlockJob*>& J, bool iso_rows, int b) {
+    std::vector<DevBuf> Om;
+    std::vector<const double2*> in;
+    std::vector<double2*> out;
+    std::vector<DevBuf*> ys;
+    std::vector<int> n;
+    for (auto* j : J) {
+        Om.emplace_back(D.pool.get(), (size_t)j->no * b * sizeof(double2));
+        launch_rand_fill(D.stream, Om.back().as<double2>(), (int64_t)j->no * b, j->seed);
+        j->Q = DevBuf(D.pool.get(), (size_t)j->ni * b * sizeof(double2));
+    }
+    for (size_t q = 0; q < J.size(); ++q) {
+        in.push_back(Om[q].as<double2>());
+        out.push_back(J[q]->Q.as<double2>());
+        ys.push_back(&J[q]->Q);
+        n.push_back(J[q]->ni);
+    }
+    block_amul(D, J, iso_rows, false, in, b, out);
+    block_orth(D, ys, n, b, 1, false, J[0]->seed ^ 0x5bd1e995u);
+}
+
+// One power step for every job, Q <- orth(A orth(A^H Q)), then P = A^H Q for Rayleigh-Ritz.
+inline void block_power(Device& D, std::vector<BlockJob*>& J, bool iso_rows, int b) {
+    std::vector<const double2*> in;
+    std::vector<double2*> out;
+    std::vector<DevBuf*> ys;
+    std::vector<int> n;
+    for (auto* j : J) {
+        j->P = DevBuf(D.pool.get(), (size_t)j->no * b * sizeof(double2));
+        in.push_back(j->Q.as<double2>());
+        out.push_back(j->P.as<double2>());
+        ys.push_back(&j->P);
+        n.push_back(j->no);
+    }
+    block_amul(D, J, iso_rows, true, in, b, out);
+    block_orth(D, ys, n, b, 1, false, J[0]->seed ^ 0x27d4eb2fu);
+    in.clear();
+    out.clear();
+    ys.clear();
+    n.clear();
+    for (auto* j : J) {
+        j->Q = DevBuf(D.pool.get(), (size_t)j->ni * b * sizeof(double2));
+        in.push_back(j->P.as<double2>());
+        out.push_back(j->Q.as<double2>());
+        ys.push_back(&j->Q);
+        n.push_back(j->ni);
+    }
+    block_amul(D, J, iso_rows, false, in, b, out);
+    block_orth(D, ys, n, b, 2, true, J[0]->seed ^ (0x9e3779b9u + 31u * (uint64_t)J[0]->power));
+    in.clear();
+    out.clear();
+    for (auto* j : J) {
+        j->P = DevBuf(D.pool.get(), (size_t)j->no * b * sizeof(double2));
+        in.push_back(j->Q.as<double2>());
+        out.push_back(j->P.as<double2>());
+        j->power += 1;
+    }
+    block_amul(D, J, iso_rows, true, in, b, out);
+}
+
+// Rayleigh-Ritz on every job's block: H = P^H P / |M|_F^2 = V Theta V^H (one batched
+// eigensolve), Z = Q V and W = P V for all b pairs, their residuals, the certified values
+// (X.s) and the bound on every other value (X.ceil).
+inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool iso_rows, int b, int mode,
+                       bool roc_trd) {
+    const int nj = (int)J.size();
+    const int kc = b;
+    std::vector<SectorSvd> H(nj);
+    std::vector<SectorSvd*> hp;
+    std::vector<GemmJob> g;
+    for (int q = 0; q < nj; ++q) {
+        H[q].k = b;
+        H[q].gram = true;
+        H[q].G = DevBuf(D.pool.get(), (size_t)b * b * sizeof(double2));
+        g.push_back(GemmJob{J[q]->P.as<double2>(), b, J[q]->P.as<double2>(), b, H[q].G.as<double2>(), b, b, J[q]->no});
+        hp.push_back(&H[q]);
+    }
+    gemm_jobs(D, 'C', 'N', g);
+    for (int q = 0; q < nj; ++q)
+        launch_zaxpby(D.stream, H[q].G.as<double2>(), H[q].G.as<double2>(), nullptr,
+                      make_double2(1.0 / J[q]->f2, 0.0), make_double2(0.0, 0.0), (int64_t)b * b);
+    {
+        EventTimer ev;
+        ev.start(D.stream);
+        gram_solve(D, sol, hp, mode, roc_trd);
+        for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
+        ev.stop(D.stream);
+        D.blk.eig_ms += ev.ms();
+        D.blk.eig_calls += 1;
+    }
+    std::vector<DevBuf> Vd;
+    for (int q = 0; q < nj; ++q) {  // rows v_j^H in descending order
+        Vd.emplace_back(D.pool.get(), (size_t)kc * b * sizeof(double2));
+        std::vector<Copy2D> cps(kc);
+        for (int j = 0; j < kc; ++j) {
+            Copy2D c{};
+            c.src_off = (int64_t)(b - 1 - j - H[q].ev_lo) * b;
+            c.dst_off = (int64_t)j * b;
+            c.rows = 1;
+            c.cols = b;
+            c.src_ld = b;
+            c.dst_ld = b;
+            cps[j] = c;
+        }
+        DevBuf dc = upload_vec(D, cps);
+        launch_copy2d<double2>(D.stream, H[q].Ev.as<double2>(), Vd.back().as<double2>(), dc.as<Copy2D>(), kc, 1,
+                               nullptr);
         D.sync();
-        const double bound = kBlockResid * theta[0];
-        cert = 0;
+    }
+    std::vector<GemmJob> gz, gw;
+    for (int q = 0; q < nj; ++q) {
+        SectorSvd& X = *J[q]->X;
+        X.Z = DevBuf(D.pool.get(), (size_t)J[q]->ni * kc * sizeof(double2));
+        X.W = DevBuf(D.pool.get(), (size_t)J[q]->no * kc * sizeof(double2));
+        gz.push_back(GemmJob{J[q]->Q.as<double2>(), b, Vd[q].as<double2>(), b, X.Z.as<double2>(), J[q]->ni, kc, b});
+        gw.push_back(GemmJob{J[q]->P.as<double2>(), b, Vd[q].as<double2>(), b, X.W.as<double2>(), J[q]->no, kc, b});
+    }
+    gemm_jobs(D, 'N', 'C', gz);
+    gemm_jobs(D, 'N', 'C', gw);
+    // residuals of the Ritz pairs: A W_j = G z_j against theta_j z_j
+    std::vector<DevBuf> GZ;
+    std::vector<const double2*> in;
+    std::vector<double2*> out;
+    for (int q = 0; q < nj; ++q) {
+        GZ.emplace_back(D.pool.get(), (size_t)J[q]->ni * kc * sizeof(double2));
+        in.push_back(J[q]->X->W.as<double2>());
+        out.push_back(GZ.back().as<double2>());
+    }
+    block_amul(D, J, iso_rows, false, in, kc, out);
+    constexpr int rows_per = 256;
+    std::vector<int> chunks(nj);
+    std::vector<size_t> off(nj + 1, 0);
+    std::vector<std::vector<double>> theta(nj, std::vector<double>(kc));
+    std::vector<double> th_all;
+    for (int q = 0; q < nj; ++q) {
+        chunks[q] = (J[q]->ni + rows_per - 1) / rows_per;
+        off[q + 1] = off[q] + (size_t)chunks[q] * kc;
+        for (int j = 0; j < kc; ++j) theta[q][j] = H[q].s[j] * H[q].s[j] * J[q]->f2;
+        th_all.insert(th_all.end(), theta[q].begin(), theta[q].end());
+    }
+    DevBuf dth = upload_vec(D, th_all);
+    DevBuf part(D.pool.get(), off[nj] * sizeof(double));
+    for (int q = 0; q < nj; ++q)
+        launch_resid_cols(D.stream, GZ[q].as<double2>(), kc, J[q]->X->Z.as<double2>(), kc,
+                          dth.as<double>() + (size_t)q * kc, J[q]->ni, kc, rows_per, part.as<double>() + off[q]);
+    std::vector<double> hpart(off[nj]);
+    KV_HIP(hipMemcpyAsync(hpart.data(), part.get(), hpart.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+    D.sync();
+    for (int q = 0; q < nj; ++q) {
+        SectorSvd& X = *J[q]->X;
+        const double bound = kBlockResid * theta[q][0];
+        int cert = 0;
         for (; cert < kc; ++cert) {
             double r2s = 0.0;
-            for (int y = 0; y < chunks; ++y) r2s += hp[(size_t)y * kc + cert];
+            for (int y = 0; y < chunks[q]; ++y) r2s += hpart[off[q] + (size_t)y * kc + cert];
             if (!(std::sqrt(r2s) <= bound)) break;
         }
-        if (cert == kc) break;
+        // the sector's Frobenius mass outside the certified pairs bounds every other value
+        double mass = J[q]->f2;
+        for (int j = 0; j < cert; ++j) mass -= theta[q][j];
+        X.ceil = std::sqrt(std::max(mass, 0.0) + kBlockMassSlack * J[q]->f2);
+        X.s.assign((size_t)cert, 0.0);
+        for (int j = 0; j < cert; ++j) X.s[j] = std::sqrt(theta[q][j]);
+        X.block = cert > 0;
+        X.iso = true;
+        X.gram = false;
+        X.colside = !iso_rows;
+        X.kc = kc;
+        X.k = cert;
+        J[q]->cert = cert;
+        J[q]->theta = theta[q];
     }
-    if (cert == 0) {
-        X.Z.reset();
-        X.W.reset();
-        return false;
-    }
-    // the sector's Frobenius mass outside the certified pairs bounds every other value
-    double mass = f2;
-    for (int j = 0; j < cert; ++j) mass -= theta[j];
-    X.ceil = std::sqrt(std::max(mass, 0.0) + kBlockMassSlack * f2);
-    X.s.assign((size_t)cert, 0.0);
-    for (int j = 0; j < cert; ++j) X.s[j] = std::sqrt(theta[j]);
-    X.block = true;
-    X.iso = true;
-    X.gram = false;
-    X.colside = !iso_rows;
-    X.kc = kc;
-    X.k = cert;
-    return true;
 }
 
 // Factors of a block-route sector: the isometry is Z's leading columns, the other factor W
@@ -1028,6 +1206,33 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
     const int np = (int)S.pairs.size();
     std::vector<SectorSvd> X(np);
     SvdResult res;
+    // full routes: the isometric-side Gram matrix when the isometry is the smaller side, the
+    // smaller side's Gram matrix (guarded below) otherwise, the exact SVD for small sectors
+    auto full_route = [&](int p) {
+        const int64_t iso_side = left2right ? X[p].nr : X[p].nc;
+        const int64_t oth_side = left2right ? X[p].nc : X[p].nr;
+        const bool big = !opt.force_exact && std::min(X[p].nr, X[p].nc) >= opt.iso_min;
+        if (big && iso_side <= oth_side)
+            gram_prepare(D, X[p], true, !left2right);
+        else if (big && cutoff >= 0.0)
+            gram_prepare(D, X[p]);
+        else
+            exact_sector(D, sol, X[p], cutoff < 0.0);
+    };
+    // the isometric Gram route on the longer side, Householder-reduced to the shorter one
+    // when it is at least twice as long
+    auto iso_route = [&](int p) {
+        const int64_t iso_side = left2right ? X[p].nr : X[p].nc;
+        const int64_t oth_side = left2right ? X[p].nc : X[p].nr;
+        if (opt.qr_reduce && iso_side >= kQrReduceRatio * oth_side) qr_reduce(D, sol, X[p], !left2right);
+        gram_prepare(D, X[p], true, !left2right);
+    };
+    // the block route's width: maxdim plus the oversampling, trimmed to kBlockEigSmall when
+    // that still holds maxdim (rocSOLVER's zheevd tridiagonalises sides up to 256 in one
+    // kernel and goes column by column above)
+    const int bw = std::max(maxdim, std::min(maxdim + kBlockOversample, kBlockEigSmall));
+    const bool block_ok = opt.block && !opt.force_exact && cutoff >= 0.0 && maxdim > 0 && bw <= kCholMax;
+    std::vector<int> cand;
     // phase 1: gather every sector matrix and compute its singular values
     for (int p = 0; p < np; ++p) {
         const SvdSector& rs = S.rows[S.pairs[p].first];
@@ -1063,24 +1268,44 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             D.sync();
         }
         // sectors at least twice as wide as the block take the block subspace route
-        const int bw = maxdim + kBlockOversample;
-        const bool blk = opt.block && !opt.force_exact && cutoff >= 0.0 && maxdim > 0 && bw <= kCholMax &&
-                         std::min(rs.len, cs.len) >= 2 * (int64_t)bw;
-        if (blk) {
-            const uint64_t seed = ((uint64_t)rs.len << 40) ^ ((uint64_t)cs.len << 16) ^ (uint64_t)p;
-            if (block_sector(D, sol, X[p], left2right, bw, maxdim + 1, seed, opt.eig_mode, opt.roc_trd)) continue;
-            res.block_fallbacks += 1;
-            D.blk.fallbacks += 1;
-        }
-        const int64_t iso_side = left2right ? rs.len : cs.len;
-        const int64_t oth_side = left2right ? cs.len : rs.len;
-        const bool big = !opt.force_exact && std::min(rs.len, cs.len) >= opt.iso_min;
-        if (big && iso_side <= oth_side)
-            gram_prepare(D, X[p], true, !left2right);  // the isometry is the smaller side
-        else if (big && cutoff >= 0.0)
-            gram_prepare(D, X[p]);  // Gram on the smaller side; guarded below
+        if (block_ok && std::min(rs.len, cs.len) >= 2 * (int64_t)bw)
+            cand.push_back(p);
         else
-            exact_sector(D, sol, X[p], cutoff < 0.0);
+            full_route(p);
+    }
+    // the block route, all its sectors together: start block, one power step, Rayleigh-Ritz
+    std::vector<BlockJob> jobs(cand.size());
+    std::vector<int> job_of(np, -1);
+    if (!cand.empty()) {
+        std::vector<BlockJob*> jp;
+        DevBuf sc(D.pool.get(), 1024 * sizeof(double)), f2d(D.pool.get(), cand.size() * sizeof(double));
+        for (size_t i = 0; i < cand.size(); ++i) {
+            const int p = cand[i];
+            BlockJob& j = jobs[i];
+            j.X = &X[p];
+            j.ni = (int)(left2right ? X[p].nr : X[p].nc);
+            j.no = (int)(left2right ? X[p].nc : X[p].nr);
+            j.seed = ((uint64_t)X[p].nr << 40) ^ ((uint64_t)X[p].nc << 16) ^ (uint64_t)p;
+            launch_nrm2sq(D.stream, X[p].M.as<double2>(), X[p].nr * X[p].nc, f2d.as<double>() + i, sc.as<double>());
+            job_of[p] = (int)i;
+            jp.push_back(&j);
+        }
+        std::vector<double> f2(cand.size());
+        KV_HIP(hipMemcpyAsync(f2.data(), f2d.get(), f2.size() * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+        D.sync();
+        for (size_t i = 0; i < cand.size(); ++i) jobs[i].f2 = f2[i] > 0 ? f2[i] : 1.0;
+        block_start(D, jp, left2right, bw);
+        block_power(D, jp, left2right, bw);
+        block_ritz(D, sol, jp, left2right, bw, opt.eig_mode, opt.roc_trd);
+        for (int p : cand)
+            if (!X[p].block) {  // no certified value: the full route
+                jobs[job_of[p]].fallback = " fallback: no certified value";
+                X[p].Z.reset();
+                X[p].W.reset();
+                full_route(p);
+                res.block_fallbacks += 1;
+                D.blk.fallbacks += 1;
+            }
     }
     // Gram eigenproblems: one call for all sectors (zheevd needs one side per call)
     {
@@ -1117,25 +1342,32 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             if (kept[p] > 0 && (s_last < 0 || X[p].s[kept[p] - 1] < s_last)) s_last = X[p].s[kept[p] - 1];
         }
         bool redo = false;
+        std::vector<BlockJob*> more;  // block sectors that take another power step
         for (int p = 0; p < np; ++p) {
             if (X[p].block) {
-                // the cut must stay above the tail, and either fall inside the certified
-                // values or leave every value beyond them (at most ceil) out as well
-                const size_t ns = X[p].s.size();
-                bool ok = !(kept[p] > 0 && X[p].s[kept[p] - 1] < opt.tail * X[p].s[0]);
-                if ((size_t)kept[p] >= ns)
-                    ok = ok && (X[p].ceil <= cutoff || (tot >= maxdim && X[p].ceil < s_last));
-                if (!ok) {
-                    X[p].block = false;
-                    X[p].Z.reset();
-                    X[p].W.reset();
-                    gram_prepare(D, X[p], true, !left2right);
-                    gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
-                    X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
-                    res.block_fallbacks += 1;
-                    D.blk.fallbacks += 1;
-                    redo = true;
+                // settled: the kept values stay above the tail, and every value beyond the
+                // certified ones (at most ceil) falls below the cutoff or, when maxdim binds,
+                // below the smallest kept value
+                const bool tail_ok = !(kept[p] > 0 && X[p].s[kept[p] - 1] < opt.tail * X[p].s[0]);
+                const bool settled = X[p].ceil <= cutoff || (tot >= maxdim && X[p].ceil < s_last);
+                if (tail_ok && settled) continue;
+                // another power step can only tighten the bound of a maxdim-bound cut
+                BlockJob& j = jobs[job_of[p]];
+                if (tail_ok && tot >= maxdim && j.power < kBlockPowerMax) {
+                    more.push_back(&j);
+                    continue;
                 }
+                // the full eigensolver on the isometric side
+                jobs[job_of[p]].fallback = tail_ok ? " fallback: power steps exhausted" : " fallback: tail";
+                X[p].block = false;
+                X[p].Z.reset();
+                X[p].W.reset();
+                iso_route(p);
+                gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
+                X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
+                res.block_fallbacks += 1;
+                D.blk.fallbacks += 1;
+                redo = true;
             } else if (X[p].iso) {
                 if (cutoff >= 0.0 && !X[p].refined && !X[p].s.empty() && s_last >= 0.0 &&
                     opt.tail * X[p].s[0] >= s_last) {
@@ -1145,24 +1377,49 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             } else if (X[p].gram && kept[p] > 0 && X[p].s[kept[p] - 1] < opt.guard * X[p].s[0]) {
                 // U = M V / s would lose orthonormality at these values: redo the sector
                 // on its isometric (larger) side
-                gram_prepare(D, X[p], true, !left2right);
+                iso_route(p);
                 gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
                 X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
                 redo = true;
             }
         }
+        if (!more.empty()) {
+            block_power(D, more, left2right, bw);
+            block_ritz(D, sol, more, left2right, bw, opt.eig_mode, opt.roc_trd);
+            for (auto* j : more)
+                if (!j->X->block) {  // lost every certified value: the full route
+                    const int p = (int)(j->X - X.data());
+                    j->fallback = " fallback: no certified value after a power step";
+                    X[p].Z.reset();
+                    X[p].W.reset();
+                    full_route(p);
+                    if (X[p].gram) gram_solve(D, sol, {&X[p]}, opt.eig_mode, opt.roc_trd);
+                    if (X[p].iso) X[p].s.resize((size_t)std::min(X[p].nr, X[p].nc));
+                    res.block_fallbacks += 1;
+                    D.blk.fallbacks += 1;
+                }
+            redo = true;
+        }
         if (!redo) break;
     }
+    for (int p = 0; p < np; ++p) res.kept_by_q[S.rows[S.pairs[p].first].q] = kept[p];
     for (int p = 0; p < np; ++p) {
         res.total += (int)X[p].s.size();
         res.refined_sectors += X[p].refined ? 1 : 0;
         char buf[160];
-        std::snprintf(buf, sizeof buf, " [%lldx%lld %s smax %.2e kept %d last %.2e]",
-                      (long long)X[p].nr, (long long)X[p].nc,
+        std::snprintf(buf, sizeof buf, " [%lldx%lld %s%s smax %.2e kept %d last %.2e]",
+                      (long long)S.rows[S.pairs[p].first].len, (long long)S.cols[S.pairs[p].second].len,
                       X[p].block ? "block" : X[p].iso ? (X[p].refined ? "iso2" : "iso") : (X[p].gram ? "gram" : "exact"),
+                      X[p].reduced ? "-qr" : "",
                       X[p].s.empty() ? 0.0 : X[p].s[0], kept[p],
                       kept[p] > 0 ? X[p].s[kept[p] - 1] : 0.0);
         res.detail += buf;
+        if (job_of[p] >= 0) {
+            const BlockJob& j = jobs[job_of[p]];
+            std::snprintf(buf, sizeof buf, "{power %d cert %d ceil %.2e%s}", j.power, j.cert,
+                          X[p].block ? X[p].ceil : 0.0, j.fallback);
+            res.detail += buf;
+        }
         if (X[p].block) {
             res.block_sectors += 1;
             D.blk.sectors += 1;
@@ -1176,6 +1433,7 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
         } else {
             res.exact_sectors += 1;
         }
+        if (X[p].reduced && kept[p] > 0) qr_expand(D, X[p], X[p].colside, kept[p]);
     }
     // phase 3: output layouts and scatter with S absorbed
     Layout Ll, Rl;

@@ -114,7 +114,7 @@ class TDVP {
             const double te = c.s();
             tm_.evl2 += te;
             Clock c2;
-            SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, true);
+            SvdResult r = bond_svd(*mm, i, true);
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
@@ -141,7 +141,7 @@ class TDVP {
             const double te = c.s();
             tm_.evl2 += te;
             Clock c2;
-            SvdResult r = svd_truncate(dev_, *sol_, *mm, 2, cfg_.cutoff, cfg_.maxdim, false);
+            SvdResult r = bond_svd(*mm, i - 1, false);
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
@@ -239,6 +239,10 @@ class TDVP {
     bool timing = true;
 
   private:
+    // The two-site bond SVD of bond (i, i+1) in one sweep direction (S to the right when l2r).
+    SvdResult bond_svd(const Tensor& mm, int /*i*/, bool l2r) {
+        return svd_truncate(dev_, *sol_, mm, 2, cfg_.cutoff, cfg_.maxdim, l2r, -1, svd_opt_);
+    }
     // Environments stay FP64 in every precision mode: they enter every later matvec, so
     // their rounding error would bias the effective Hamiltonian itself (they cost ~5% of
     // the H_eff work).
@@ -267,6 +271,7 @@ class TDVP {
     double env_flop_ = 0;
     bool verbose_ = false;
     int last_iters_ = 0;
+    SvdOptions svd_opt_ = SvdOptions::from_env();
 };
 
 // Value of the single 1x1x1 (or 1x1) closing block of a finished environment pass.

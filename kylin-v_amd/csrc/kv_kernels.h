@@ -65,6 +65,8 @@ void launch_f32_to_f64(hipStream_t st, const float2* src, double2* dst, double s
 
 // A[i ld + i] = (v, 0) for i in [from, to) (the diagonal of a square matrix).
 void launch_set_diag(hipStream_t st, double2* A, int64_t ld, int from, int to, double v);
+// dst (n x n, ld n) = the lower (j <= i) or upper (j >= i) triangle of src (ld lds), the rest zero.
+void launch_tri_copy(hipStream_t st, const double2* src, int64_t lds, double2* dst, int n, bool lower);
 
 // Back-transformation of a lower-triangle Householder tridiagonalisation (LAPACK layout:
 // reflector i stored below the subdiagonal of column i, tau_i separately).
@@ -96,12 +98,27 @@ void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, doub
 // Block subspace route of the bond SVD.
 // p[e] = uniform [-1, 1)^2 complex value determined by (seed, e).
 void launch_rand_fill(hipStream_t st, double2* p, int64_t n, uint64_t seed);
-// Largest side of the Cholesky / triangular-inverse kernels.
+// Largest side of the Cholesky / triangular-inverse kernels; panel width and largest side of
+// the blocked Cholesky and its row-parallel triangular solve (16 x side complex in LDS).
 constexpr int kCholMax = 1024;
+constexpr int kCholNB = 16;
+constexpr int kCholBlockedMax = 512;
 // In place upper Cholesky factor of `batch` row-major Hermitian matrices (side sizes[b], ld,
-// stride); pivots <= tau x the largest diagonal drop their column (live[b ld + k] = 0).
+// stride); pivots <= tau x the largest diagonal drop their column (live[b ld + k] = 0);
+// dmax[b] (if not null) receives that largest diagonal.
 void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
-                       int batch);
+                       double* dmax, int batch);
+// The same factorisation blocked by 16-column panels (side <= kCholBlockedMax); Rd[b rd_stride +
+// p 256 + i 16 + j] receives the inverse of diagonal block p (zero for dropped indices).
+void launch_chol_blocked(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau,
+                         int* live, double* dmax, double2* Rd, int64_t rd_stride, int batch);
+// Q = Y R^{-1} (Y n x m row-major; R upper, ld ldr, and Rd from launch_chol_blocked); m <= 512.
+void launch_trsm_rows(hipStream_t st, const double2* Y, double2* Q, int n, int m, const double2* R, int ldr,
+                      const double2* Rd);
+// Columns c < ld of Y (n x ld, row-major) with live[c] == 0 refilled with seeded random values
+// whose squared column norm is about dmax[0].
+void launch_refill_cols(hipStream_t st, double2* Y, int n, int ld, const int* live, const double* dmax,
+                        uint64_t seed);
 // T = R^{-1} over the live columns (dropped rows and columns zero); nmax = largest side.
 void launch_tri_inv(hipStream_t st, const double2* R, double2* T, const int* sizes, const int* live, int ld,
                     int64_t stride, int nmax, int batch);

@@ -988,6 +988,25 @@ void launch_set_diag(hipStream_t st, double2* A, int64_t ld, int from, int to, d
     KV_HIP(hipGetLastError());
 }
 
+// dst (n x n, row-major, ld n) = one triangle of src (ld lds), the other triangle zero:
+// dst[i][j] = src[i * lds + j] where j <= i (lower) or j >= i (upper).
+__global__ __launch_bounds__(256) void tri_copy_kernel(const double2* __restrict__ src, int64_t lds,
+                                                       double2* __restrict__ dst, int n, int lower) {
+    const int64_t total = (int64_t)n * n;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < total; e += (int64_t)gridDim.x * 256) {
+        const int i = (int)(e / n), j = (int)(e % n);
+        const bool keep = lower ? j <= i : j >= i;
+        dst[e] = keep ? src[(int64_t)i * lds + j] : make_double2(0.0, 0.0);
+    }
+}
+
+void launch_tri_copy(hipStream_t st, const double2* src, int64_t lds, double2* dst, int n, bool lower) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(tri_copy_kernel, dim3(ew_blocks((int64_t)n * n)), dim3(256), 0, st, src, lds, dst, n,
+                       lower ? 1 : 0);
+    KV_HIP(hipGetLastError());
+}
+
 // ---- back-transformation of a Householder tridiagonalisation ------------------------
 // Lower-triangle reduction Q^H A Q = T of a column-major n x n matrix: reflector i < n-1 is
 // H_i = I - tau_i v_i v_i^H with v_i[r] = 0 for r <= i, 1 at r = i + 1 and A[i n + r] below.
@@ -1526,7 +1545,7 @@ __device__ inline double wave_max(double v) {
 // live[b ld + k] = 1 for a live column.  n = sizes[b] <= kCholMax.
 __global__ __launch_bounds__(1024) void chol_upper_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
                                                           int ld, int64_t stride, double tau,
-                                                          int* __restrict__ live) {
+                                                          int* __restrict__ live, double* __restrict__ dmax_out) {
     const int b = blockIdx.x, n = sizes[b];
     S += (int64_t)b * stride;
     live += (int64_t)b * ld;
@@ -1544,6 +1563,7 @@ __global__ __launch_bounds__(1024) void chol_upper_kernel(double2* __restrict__ 
         double t = 0.0;
         for (int w = 0; w < 16; ++w) t = fmax(t, red[w]);
         dmax_s = t;
+        if (dmax_out) dmax_out[b] = t;
     }
     __syncthreads();
     const double thr = tau * dmax_s;
@@ -1579,6 +1599,198 @@ __global__ __launch_bounds__(1024) void chol_upper_kernel(double2* __restrict__ 
         }
         __syncthreads();
     }
+}
+
+__device__ inline double2 cfma_conj(double2 acc, double2 a, double2 b) {  // acc + conj(a) b
+    return make_double2(acc.x + a.x * b.x + a.y * b.y, acc.y + a.x * b.y - a.y * b.x);
+}
+__device__ inline double2 cfma(double2 acc, double2 a, double2 b) {  // acc + a b
+    return make_double2(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
+}
+
+// The same factorisation blocked by panels of kCholNB = 16 columns, for sides up to
+// kCholBlockedMax, one 512-thread workgroup per matrix: wave 0 factors the diagonal block in LDS (the same pivot drops) and
+// inverts it over its live indices; every thread then forms one column of the panel rows
+// R_{p,j} = R_pp^{-H} S_{p,j} (kept in LDS), and the trailing triangle takes the rank-16
+// update S_ij -= sum_l conj(R_li) R_lj with one read-modify-write per element per panel.  Rd
+// receives each diagonal block's inverse (16 x 16 row-major, zero rows and columns for dropped
+// indices) for trsm_rows_kernel.
+constexpr int kCholBlockedThreads = 512;
+__global__ __launch_bounds__(kCholBlockedThreads) void chol_blocked_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
+                                                            int ld, int64_t stride, double tau,
+                                                            int* __restrict__ live, double* __restrict__ dmax_out,
+                                                            double2* __restrict__ Rd, int64_t rd_stride) {
+    constexpr int NB = kCholNB, NT = kCholBlockedThreads, NW = NT / 64;
+    const int b = blockIdx.x, n = sizes[b];
+    S += (int64_t)b * stride;
+    live += (int64_t)b * ld;
+    Rd += (int64_t)b * rd_stride;
+    __shared__ double2 Dg[NB][NB + 1], Di[NB][NB + 1];
+    __shared__ double2 panel[NB][kCholBlockedMax];
+    __shared__ double red[NW];
+    __shared__ double thr_s;
+    __shared__ int lv_s[NB];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double2 zero2 = make_double2(0.0, 0.0);
+    double mx = 0.0;
+    for (int i = tid; i < n; i += NT) mx = fmax(mx, S[(int64_t)i * ld + i].x);
+    mx = wave_max(mx);
+    if (lane == 0) red[wv] = mx;
+    __syncthreads();
+    if (tid == 0) {
+        double t = 0.0;
+        for (int w = 0; w < NW; ++w) t = fmax(t, red[w]);
+        thr_s = tau * t;
+        if (dmax_out) dmax_out[b] = t;
+    }
+    __syncthreads();
+    const double thr = thr_s;
+    for (int p = 0; p < n; p += NB) {
+        const int w = min(NB, n - p);
+        if (wv == 0) {  // diagonal block: factor and invert in LDS; one wave's LDS ops stay in order
+            for (int e = lane; e < NB * NB; e += 64) {
+                Dg[e / NB][e % NB] = zero2;
+                Di[e / NB][e % NB] = zero2;
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int e = lane; e < w * w; e += 64) {  // (a select of a load here takes a stack slot)
+                const int i = e / w, j = e % w;
+                if (j >= i) Dg[i][j] = S[(int64_t)(p + i) * ld + p + j];
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int k = 0; k < w; ++k) {
+                const double piv = Dg[k][k].x;
+                const bool lk = piv > thr && piv > 0.0;  // NaN is not live
+                const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
+                __builtin_amdgcn_wave_barrier();
+                if (lane > k && lane < w) Dg[k][lane] = make_double2(Dg[k][lane].x * ri, Dg[k][lane].y * ri);
+                if (lane == k) {
+                    Dg[k][k] = make_double2(lk ? sqrt(piv) : 0.0, 0.0);
+                    lv_s[k] = lk ? 1 : 0;
+                }
+                __builtin_amdgcn_wave_barrier();
+                if (lk)
+                    for (int e = lane; e < NB * NB; e += 64) {
+                        const int i = e / NB, j = e % NB;
+                        if (i > k && j >= i && j < w) {
+                            const double2 a = Dg[k][i], c = Dg[k][j];
+                            Dg[i][j] = make_double2(Dg[i][j].x - (a.x * c.x + a.y * c.y),
+                                                    Dg[i][j].y - (a.x * c.y - a.y * c.x));
+                        }
+                    }
+                __builtin_amdgcn_wave_barrier();
+            }
+            if (lane < w && lv_s[lane]) {  // column `lane` of the inverse, by back substitution
+                const int c = lane;
+                Di[c][c] = make_double2(1.0 / Dg[c][c].x, 0.0);
+                for (int i = c - 1; i >= 0; --i) {
+                    double2 acc = zero2;
+                    if (lv_s[i]) {
+                        for (int k = i + 1; k <= c; ++k) acc = cfma(acc, Dg[i][k], Di[k][c]);
+                        acc = make_double2(-acc.x / Dg[i][i].x, -acc.y / Dg[i][i].x);
+                    }
+                    Di[i][c] = acc;
+                }
+            }
+            __builtin_amdgcn_wave_barrier();
+            for (int e = lane; e < NB * NB; e += 64) {
+                const int i = e / NB, j = e % NB;
+                if (i < w && j < w && j >= i) S[(int64_t)(p + i) * ld + p + j] = Dg[i][j];
+                Rd[(int64_t)(p / NB) * NB * NB + e] = Di[i][j];
+            }
+            if (lane < w) live[p + lane] = lv_s[lane];
+        }
+        __syncthreads();
+        const int t0 = p + w, mt = n - t0;
+        for (int c = tid; c < mt; c += NT) {  // panel rows, one column each: y = Rinv_pp^H x
+            double2 y[NB];
+#pragma unroll
+            for (int l = 0; l < NB; ++l) y[l] = zero2;
+            // m stays a loop (the compiler would otherwise hoist all of Di into registers) and l
+            // runs over every slot with a predicate, so y stays in registers
+#pragma unroll 1
+            for (int m = 0; m < w; ++m) {
+                const double2 xm = S[(int64_t)(p + m) * ld + t0 + c];
+#pragma unroll
+                for (int l = 0; l < NB; ++l)
+                    if (l >= m) y[l] = cfma_conj(y[l], Di[m][l], xm);
+            }
+#pragma unroll
+            for (int l = 0; l < NB; ++l)
+                if (l < w) {
+                    S[(int64_t)(p + l) * ld + t0 + c] = y[l];
+                    panel[l][c] = y[l];
+                }
+        }
+        __syncthreads();
+        for (int r = wv; r < mt; r += NW) {  // trailing triangle: row t0 + r per wave
+            double2 a[NB];
+#pragma unroll
+            for (int l = 0; l < NB; ++l) a[l] = l < w ? panel[l][r] : zero2;
+            double2* row = S + (int64_t)(t0 + r) * ld + t0;
+            for (int c = r + lane; c < mt; c += 64) {
+                double2 acc = zero2;
+#pragma unroll
+                for (int l = 0; l < NB; ++l) acc = cfma_conj(acc, a[l], panel[l][c]);
+                const double2 v = row[c];
+                row[c] = make_double2(v.x - acc.x, v.y - acc.y);
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// Q = Y R^{-1} (Y n x m row-major, R the upper factor of chol_blocked_kernel in S with ld ldr,
+// Rd its diagonal blocks' inverses), rows in parallel: 16 rows per workgroup kept in LDS, and
+// per panel p of 16 columns Q_p = (Y_p - Q_{<p} R_{<p,p}) Rinv_pp.  grid ceil(n / 16).
+__global__ __launch_bounds__(256) void trsm_rows_kernel(const double2* __restrict__ Y, double2* __restrict__ Q,
+                                                        int n, int m, const double2* __restrict__ R, int ldr,
+                                                        const double2* __restrict__ Rd) {
+    constexpr int NB = kCholNB;
+    __shared__ double2 q[NB][kCholBlockedMax];
+    __shared__ double2 t[NB][NB + 1];
+    const int tr = threadIdx.x / NB, tc = threadIdx.x % NB;
+    const int row = blockIdx.x * NB + tr;
+    const double2 zero2 = make_double2(0.0, 0.0);
+    for (int p = 0; p < m; p += NB) {
+        const int w = min(NB, m - p);
+        double2 u = zero2;
+        if (row < n && tc < w) {
+            u = Y[(int64_t)row * m + p + tc];
+            double2 acc = zero2;
+            for (int k = 0; k < p; ++k) acc = cfma(acc, q[tr][k], R[(int64_t)k * ldr + p + tc]);
+            u = make_double2(u.x - acc.x, u.y - acc.y);
+        }
+        t[tr][tc] = u;
+        __syncthreads();
+        double2 x = zero2;
+        if (tc < w) {
+            const double2* rd = Rd + (int64_t)(p / NB) * NB * NB;
+            for (int l = 0; l <= tc; ++l) x = cfma(x, t[tr][l], rd[l * NB + tc]);
+            q[tr][p + tc] = x;
+            if (row < n) Q[(int64_t)row * m + p + tc] = x;
+        }
+        __syncthreads();
+    }
+}
+
+void launch_chol_blocked(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau,
+                         int* live, double* dmax, double2* Rd, int64_t rd_stride, int batch) {
+    KV_REQUIRE(ld <= kCholBlockedMax && batch >= 1, ERR_INTERNAL, "chol_blocked: side above kCholBlockedMax");
+    KV_REQUIRE(rd_stride >= (int64_t)((ld + kCholNB - 1) / kCholNB) * kCholNB * kCholNB, ERR_INTERNAL,
+               "chol_blocked: diagonal-block buffer too small");
+    hipLaunchKernelGGL(chol_blocked_kernel, dim3(batch), dim3(kCholBlockedThreads), 0, st, S, sizes, ld, stride, tau,
+                       live, dmax, Rd, rd_stride);
+    KV_HIP(hipGetLastError());
+}
+
+void launch_trsm_rows(hipStream_t st, const double2* Y, double2* Q, int n, int m, const double2* R, int ldr,
+                      const double2* Rd) {
+    KV_REQUIRE(m <= kCholBlockedMax && m <= ldr, ERR_INTERNAL, "trsm_rows: more columns than the LDS row block holds");
+    if (n <= 0 || m <= 0) return;
+    hipLaunchKernelGGL(trsm_rows_kernel, dim3((n + kCholNB - 1) / kCholNB), dim3(kCholNB * kCholNB), 0, st, Y, Q, n, m,
+                       R, ldr, Rd);
+    KV_HIP(hipGetLastError());
 }
 
 // T = R^{-1} over the live columns of an upper factor from chol_upper_kernel (row-major, rows
@@ -1635,9 +1847,33 @@ __global__ __launch_bounds__(256) void tri_inv_kernel(const double2* __restrict_
 }
 
 void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
-                       int batch) {
+                       double* dmax, int batch) {
     KV_REQUIRE(ld <= kCholMax && batch >= 1, ERR_INTERNAL, "chol_upper: side above kCholMax");
-    hipLaunchKernelGGL(chol_upper_kernel, dim3(batch), dim3(1024), 0, st, S, sizes, ld, stride, tau, live);
+    hipLaunchKernelGGL(chol_upper_kernel, dim3(batch), dim3(1024), 0, st, S, sizes, ld, stride, tau, live, dmax);
+    KV_HIP(hipGetLastError());
+}
+
+// Column c of Y (n x ld, row-major) for every c < ld with live[c] == 0: seeded uniform
+// complex values scaled so the column's squared norm is about dmax[0] (E|u|^2 = 2/3).
+__global__ __launch_bounds__(256) void refill_cols_kernel(double2* __restrict__ Y, int n, int ld,
+                                                          const int* __restrict__ live,
+                                                          const double* __restrict__ dmax, uint64_t seed) {
+    const uint64_t s = mix64(seed);
+    const double f = sqrt(1.5 * fmax(dmax[0], 0.0) / (double)n);
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < (int64_t)n * ld; e += (int64_t)gridDim.x * 256) {
+        const int c = (int)(e % ld);
+        if (live[c]) continue;
+        const uint64_t a = mix64(s ^ (uint64_t)(2 * e)), b = mix64(s ^ (uint64_t)(2 * e + 1));
+        Y[e] = make_double2(f * ((double)(a >> 11) * 0x1.0p-52 - 1.0), f * ((double)(b >> 11) * 0x1.0p-52 - 1.0));
+    }
+}
+
+void launch_refill_cols(hipStream_t st, double2* Y, int n, int ld, const int* live, const double* dmax,
+                        uint64_t seed) {
+    const int64_t total = (int64_t)n * ld;
+    if (total <= 0) return;
+    const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
+    hipLaunchKernelGGL(refill_cols_kernel, dim3(blocks), dim3(256), 0, st, Y, n, ld, live, dmax, seed);
     KV_HIP(hipGetLastError());
 }
 

@@ -3,10 +3,11 @@
 The device route (randomised range finder with power steps and Rayleigh-Ritz) has no
 counterpart in the reference, which calls LAPACK's full SVD per sector.  This restates its
 steps in numpy — Cholesky QR with pivots at or below 1e-12 x the largest dropped
-(right-looking, the order chol_upper_kernel eliminates in), the inverse of the factor by
-back substitution (tri_inv_kernel), the compaction of live columns, the power steps
+(right-looking, the order chol_upper_kernel eliminates in; chol_blocked_kernel's 16-column
+panels with the row-parallel solve trsm_rows_kernel give the same factor and basis), the
+inverse of the factor by back substitution (tri_inv_kernel), the compaction of live columns, the power steps
 Q -> orth(A^H Q) -> orth(A P), the Ritz pairs, their residuals and the ceiling on the values
-beyond the certified ones — and checks them against LAPACK (numpy.linalg.svd):
+beyond the certified ones, and the check that settles a cut — and checks them against LAPACK (numpy.linalg.svd):
 orthonormality of the Cholesky QR bases, the certified singular values, the isometry and
 the truncated product.  Planted slips (no second Cholesky QR pass, no power step) must fail.
 """
@@ -52,10 +53,75 @@ def tri_inv(R, live):
     return T
 
 
-def block_orth(Y, passes=2):
+def chol_blocked(S, nb=16, tau=DROP):
+    """chol_blocked_kernel's order: per panel of nb columns the diagonal block's factor (with
+    the same pivot drops) and inverse, the panel rows R_pp^{-H} S_pj, the rank-nb trailing
+    update.  Returns (R, live, inverses of the diagonal blocks)."""
+    S = np.array(S, dtype=complex)
+    n = S.shape[0]
+    thr = tau * max(np.real(np.diag(S)).max(), 0.0)
+    R = np.zeros_like(S)
+    live = np.zeros(n, dtype=bool)
+    Rd = []
+    for p in range(0, n, nb):
+        w = min(nb, n - p)
+        Dg = np.triu(S[p:p + w, p:p + w])
+        lv = np.zeros(w, dtype=bool)
+        for k in range(w):
+            piv = Dg[k, k].real
+            lk = piv > thr and piv > 0
+            lv[k] = lk
+            ri = 1 / np.sqrt(piv) if lk else 0.0
+            Dg[k, k + 1:] *= ri
+            Dg[k, k] = np.sqrt(piv) if lk else 0.0
+            if lk:
+                for i in range(k + 1, w):
+                    Dg[i, i:] -= np.conj(Dg[k, i]) * Dg[k, i:]
+        Di = np.zeros((nb, nb), dtype=complex)
+        for c in range(w):
+            if not lv[c]:
+                continue
+            Di[c, c] = 1 / Dg[c, c].real
+            for i in range(c - 1, -1, -1):
+                if lv[i]:
+                    Di[i, c] = -(Dg[i, i + 1:c + 1] @ Di[i + 1:c + 1, c]) / Dg[i, i].real
+        Rd.append(Di)
+        live[p:p + w] = lv
+        R[p:p + w, p:p + w] = np.triu(Dg)
+        t0 = p + w
+        panel = Di[:w, :w].conj().T @ S[p:p + w, t0:]
+        R[p:p + w, t0:] = panel
+        S[t0:, t0:] -= panel.conj().T @ panel
+    return R, live, Rd
+
+
+def trsm_rows(Y, R, Rd, nb=16):
+    """trsm_rows_kernel's order: per panel Q_p = (Y_p - Q_{<p} R_{<p,p}) Rinv_pp."""
+    n, m = Y.shape
+    Q = np.zeros_like(Y)
+    for p in range(0, m, nb):
+        w = min(nb, m - p)
+        u = Y[:, p:p + w] - Q[:, :p] @ R[:p, p:p + w]
+        Q[:, p:p + w] = u @ Rd[p // nb][:w, :w]
+    return Q
+
+
+def block_orth(Y, passes=2, refill=False, seed=99):
+    """Cholesky QR passes; dependent columns come out zero, or with `refill` are replaced by
+    random vectors at the largest column's scale and the pass repeats (at most twice)."""
+    rng = np.random.default_rng(seed)
     for _ in range(passes):
-        R, live = chol_upper(Y.conj().T @ Y)
-        Y = (Y @ tri_inv(R, live))[:, live]
+        for attempt in range(3):
+            S = Y.conj().T @ Y
+            R, live = chol_upper(S)
+            if live.all() or not refill or attempt == 2:
+                break
+            scale = np.sqrt(1.5 * np.real(np.diag(S)).max() / Y.shape[0])
+            dead = ~live
+            Y = Y.copy()
+            Y[:, dead] = scale * (rng.uniform(-1, 1, (Y.shape[0], dead.sum()))
+                                  + 1j * rng.uniform(-1, 1, (Y.shape[0], dead.sum())))
+        Y = Y @ tri_inv(R, live)
     return Y
 
 
@@ -64,39 +130,42 @@ def rand_block(rows, cols, seed):
     return rng.uniform(-1, 1, (rows, cols)) + 1j * rng.uniform(-1, 1, (rows, cols))
 
 
-def block_sector(M, maxdim, iso_rows=True, seed=0, power_max=3, passes=2):
-    """The device's block_sector: (certified singular values, Z isometric vectors,
-    W = A^H Z, residuals, ceiling on every other value)."""
+def block_sector(M, maxdim, iso_rows=True, seed=0, power_max=3, b=None, need=None):
+    """The device's block route for one sector: start block (one Cholesky QR pass), power
+    steps (A^H Q one pass, A P two passes with refill), Rayleigh-Ritz, residuals and the
+    ceiling.  Power steps stop once the leading min(b, maxdim + 1) pairs are certified (the
+    device decides from the cut; `need` stands in for it).  Returns (certified singular
+    values, Z isometric vectors, W = A^H Z, residuals, ceiling on every other value)."""
     A = M if iso_rows else M.conj().T
-    b = maxdim + OVERSAMPLE
-    Q = block_orth(A @ rand_block(A.shape[1], b, seed), passes)
+    b = b or maxdim + OVERSAMPLE
+    Q = block_orth(A @ rand_block(A.shape[1], b, seed), passes=1)
     f2 = np.vdot(M, M).real
     for q in range(max(power_max, 1)):
         if power_max > 0:
-            P = block_orth(A.conj().T @ Q, passes)
-            Q = block_orth(A @ P, passes)
+            P = block_orth(A.conj().T @ Q, passes=1)
+            Q = block_orth(A @ P, passes=2, refill=True, seed=seed + q)
         P = A.conj().T @ Q
         w, V = np.linalg.eigh((P.conj().T @ P) / f2)
         w, V = w[::-1], V[:, ::-1]
-        kc = min(Q.shape[1], maxdim + 1)
+        kc = b
         theta = np.maximum(w[:kc], 0) * f2
         Z, W = Q @ V[:, :kc], P @ V[:, :kc]
         res = np.linalg.norm(A @ W - Z * theta[None, :], axis=0)
         cert = 0
         while cert < kc and res[cert] <= RESID * theta[0]:
             cert += 1
-        if cert == kc:
+        if cert >= (need or kc):
             break
     ceil = np.sqrt(max(f2 - theta[:cert].sum(), 0.0) + 1e-13 * f2)
     return np.sqrt(theta[:cert]), Z, W, res, ceil
 
 
 def cut_ok(sv, ceil, kept, cutoff, maxdim, total_kept, s_last, tail=1e-5):
-    """svd_truncate's check on a block-route sector after the global cut."""
-    ok = not (kept > 0 and sv[kept - 1] < tail * sv[0])
-    if kept >= len(sv):
-        ok = ok and (ceil <= cutoff or (total_kept >= maxdim and ceil < s_last))
-    return ok
+    """svd_truncate's check on a block-route sector after the global cut: the kept values
+    stay above the tail, and every value beyond the certified ones falls below the cutoff
+    or, when maxdim binds, below the smallest kept value."""
+    tail_ok = not (kept > 0 and sv[kept - 1] < tail * sv[0])
+    return tail_ok and (ceil <= cutoff or (total_kept >= maxdim and ceil < s_last))
 
 
 def spectrum_matrix(rng, nr, nc, s):
@@ -115,12 +184,36 @@ def test_cholesky_qr_with_dropped_columns_matches_lapack_qr():
     assert not live[7] and not live[30] and live[11] and live.sum() == 38
     T = tri_inv(R, live)
     np.testing.assert_allclose(R[np.ix_(live, live)] @ T[np.ix_(live, live)], np.eye(38), atol=1e-10)
-    Q = block_orth(Y)
-    assert Q.shape == (300, 38)
-    np.testing.assert_allclose(Q.conj().T @ Q, np.eye(38), atol=1e-13)
+    Q = block_orth(Y)  # dependent columns come out zero
+    assert np.abs(Q[:, ~live]).max() == 0.0
+    Ql = Q[:, live]
+    np.testing.assert_allclose(Ql.conj().T @ Ql, np.eye(38), atol=1e-13)
     # the span is LAPACK's QR span of the live columns
     q2, _ = np.linalg.qr(Y[:, live])
-    np.testing.assert_allclose(np.abs(np.linalg.svd(q2.conj().T @ Q, compute_uv=False)), 1.0, atol=1e-12)
+    np.testing.assert_allclose(np.abs(np.linalg.svd(q2.conj().T @ Ql, compute_uv=False)), 1.0, atol=1e-12)
+    Qr = block_orth(Y, refill=True)  # or are refilled: 40 orthonormal columns
+    np.testing.assert_allclose(Qr.conj().T @ Qr, np.eye(40), atol=1e-13)
+
+
+def test_blocked_cholesky_and_row_solve_match_the_unblocked_factor():
+    """The panel-blocked factorisation and the row-parallel solve give the unblocked factor
+    R and Q = Y R^{-1} over the live columns, with dependent columns inside a panel (7, 20)
+    and on a panel edge (31), and a side (45) that is not a multiple of the panel."""
+    rng = np.random.default_rng(13)
+    Y = rng.standard_normal((200, 45)) + 1j * rng.standard_normal((200, 45))
+    Y[:, 7] = Y[:, 3] - 2j * Y[:, 5]
+    Y[:, 20] = 0.5 * Y[:, 19]
+    Y[:, 31] = Y[:, 0] + Y[:, 30]
+    S = Y.conj().T @ Y
+    R1, live1 = chol_upper(S)
+    R2, live2, Rd = chol_blocked(S)
+    assert np.array_equal(live1, live2) and live1.sum() == 42
+    np.testing.assert_allclose(R2, R1, atol=1e-12 * np.abs(R1).max())
+    Q = trsm_rows(Y, R2, Rd)
+    np.testing.assert_allclose(Q, Y @ tri_inv(R1, live1), atol=1e-12)
+    assert np.abs(Q[:, ~live1]).max() == 0.0
+    Ql = Q[:, live1]
+    np.testing.assert_allclose(Ql.conj().T @ Ql, np.eye(42), atol=1e-12)
 
 
 def test_one_cholesky_qr_pass_is_not_enough_on_ill_conditioned_blocks():
@@ -135,52 +228,59 @@ def test_one_cholesky_qr_pass_is_not_enough_on_ill_conditioned_blocks():
 @pytest.mark.parametrize("iso_rows", [True, False])
 def test_block_route_matches_lapack_on_a_bond_like_spectrum(iso_rows):
     """Sector of 800 x 1100 whose values fall like an evolved two-site tensor's: 60 at
-    O(1..1e-2), then a cliff to 1e-7 and below.  Cut at maxdim 60: the certified values
-    stop at the cliff, the ceiling (the mass beyond it, ~1e-7, plus a rounding allowance of
-    sqrt(1e-13) |M|_F) lies below the cut, and values, isometry and
-    product match LAPACK.  Cut at maxdim 64 with cutoff 1e-10: values of 1e-7 must be kept,
-    the ceiling is above the cutoff, and the sector has to go to the full route."""
+    O(1..1e-2), then a cliff to 1e-7 and below.  Cut at maxdim 60 with a block of 60 + 32:
+    the cliff values below the Cholesky QR's resolution leave the block and come back as
+    random directions, the certified values and the ceiling (the mass beyond the certified
+    pairs, ~1e-7, plus a rounding allowance of sqrt(1e-13) |M|_F) settle the cut, and values,
+    isometry and product match LAPACK.  Cut at maxdim 64 with cutoff 1e-10: values of 1e-7
+    must be kept, which the ceiling cannot settle, so the sector goes to the full route."""
     rng = np.random.default_rng(3)
     s = np.concatenate([np.logspace(0, -2, 60), 1e-7 * np.logspace(0, -6, 200)])
     M = spectrum_matrix(rng, 800, 1100, s)
     sv, Z, W, res, ceil = block_sector(M, 60, iso_rows)
     ref = np.linalg.svd(M, compute_uv=False)
     k = 60
-    assert len(sv) == k and 1e-7 < ceil < 1e-6  # the mass beyond the cliff plus rounding slack
-    assert cut_ok(sv, ceil, k, 1e-10, 60, k, sv[-1])
-    np.testing.assert_allclose(sv, ref[:k], rtol=1e-9)
-    np.testing.assert_allclose(Z[:, :k].conj().T @ Z[:, :k], np.eye(k), atol=1e-13)
+    assert len(sv) >= k and ceil < 1e-6
+    assert cut_ok(sv, ceil, k, 1e-10, 60, k, sv[k - 1])
+    np.testing.assert_allclose(sv[:k], ref[:k], rtol=1e-9)
+    np.testing.assert_allclose(Z.conj().T @ Z, np.eye(Z.shape[1]), atol=1e-13)
     rec = Z[:, :k] @ W[:, :k].conj().T if iso_rows else (Z[:, :k] @ W[:, :k].conj().T).conj().T
     u, s_, vh = np.linalg.svd(M, full_matrices=False)
     ref_rec = (u[:, :k] * s_[None, :k]) @ vh[:k]
     assert np.abs(rec - ref_rec).max() <= 1e-11 * np.abs(ref_rec).max()
     sv, Z, W, res, ceil = block_sector(M, 64, iso_rows)
-    assert len(sv) == k and not cut_ok(sv, ceil, k, 1e-10, 64, k, sv[-1])
+    assert not cut_ok(sv, ceil, 64, 1e-10, 64, 64 if len(sv) >= 64 else len(sv), sv[min(63, len(sv) - 1)])
 
 
 def test_block_route_power_steps_until_certified():
     """Seventy values from 1 to 0.7, then a tail from 1e-5: the range finder alone leaves
-    Ritz residuals far above 1e-13 of the largest value, one power step brings the leading
-    maxdim + 1 pairs under it; the values then match LAPACK and the next value is certified."""
+    Ritz residuals far above 1e-13 of the largest value, one power step brings every pair of
+    the block of 96 under it, the flat top included, so the mass beyond them is the tail's
+    and the cut at 64 is settled; the values match LAPACK."""
     rng = np.random.default_rng(4)
     s = np.concatenate([1.0 - 0.3 * np.arange(70) / 70, 1e-5 * np.exp(-np.arange(500) / 30.0)])
     M = spectrum_matrix(rng, 900, 1000, s)
-    sv0, _, _, _, _ = block_sector(M, 64, power_max=0)
-    sv1, _, _, _, _ = block_sector(M, 64, power_max=1)
+    sv0, _, _, _, c0 = block_sector(M, 64, power_max=0)
+    sv1, _, _, _, c1 = block_sector(M, 64, power_max=1, need=70)
     ref = np.linalg.svd(M, compute_uv=False)
     assert len(sv0) < 65 <= len(sv1)
     np.testing.assert_allclose(sv1[:65], ref[:65], rtol=1e-10)
-    assert cut_ok(sv1, 0.0, 64, 1e-10, 64, 64, sv1[63])
+    if len(sv0):  # with nothing certified the device goes to the full route at once
+        assert not cut_ok(sv0, c0, min(64, len(sv0)), 1e-10, 64, 64, sv0[min(63, len(sv0) - 1)])
+    assert cut_ok(sv1, c1, 64, 1e-10, 64, 64, sv1[63])
 
 
-def test_block_route_rank_deficient_sector_drops_columns():
-    """Rank 40 < block 96: the start block loses its dependent columns and the certified
-    values stop at the rank, with a ceiling at the rounding level, so a cut that keeps all
-    forty is certified."""
+def test_block_route_rank_deficient_sector_refills_columns():
+    """Rank 40 < block 96: the range finder's dependent columns drop out, the final basis
+    refills them with random directions, so it keeps 96 orthonormal columns; the forty
+    values match LAPACK, the rest are null (at rounding), and the ceiling sits at the rounding
+    allowance, so a cut at any cutoff above it is settled."""
     rng = np.random.default_rng(5)
     M = spectrum_matrix(rng, 600, 700, np.logspace(0, -3, 40))
     sv, Z, W, res, ceil = block_sector(M, 64)
-    assert Z.shape[1] <= 41 and len(sv) >= 40
+    assert Z.shape[1] == 96 and len(sv) >= 40
+    np.testing.assert_allclose(Z.conj().T @ Z, np.eye(96), atol=1e-13)
     ref = np.linalg.svd(M, compute_uv=False)[:40]
     np.testing.assert_allclose(sv[:40], ref, rtol=1e-9)
-    assert ceil < 1e-6 and cut_ok(sv, ceil, len(sv), 1e-6, 64, len(sv), sv[-1])
+    assert np.all(sv[40:] < 1e-6 * sv[0])
+    assert ceil < 1e-6 and cut_ok(sv, ceil, 40, 1e-6, 64, 40, sv[39])
