@@ -944,15 +944,11 @@ inline void block_amul(Device& D, std::vector<BlockJob*>& J, bool iso_rows, bool
 inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int>& n, int b, int passes,
                        bool refill, uint64_t seed) {
     const int nj = (int)Y.size();
-    KV_REQUIRE(b <= kCholMax, ERR_INTERNAL, "block route: block wider than the Cholesky kernel");
-    // blocks up to kCholBlockedMax: panel-blocked Cholesky and a row-parallel triangular solve;
-    // wider: the unblocked factor, its inverse and a GEMM
-    const bool blocked = b <= kCholBlockedMax;
+    KV_REQUIRE(b <= kCholMax, ERR_INTERNAL, "block route: block wider than kCholMax");
+    // S_j = Y_j^H Y_j (one grouped GEMM), the left-looking panel Cholesky with T_j = R_j^{-1}
+    // (one launch per 32 columns, all jobs together), Q_j = Y_j T_j (one grouped GEMM)
     const size_t bb = (size_t)b * b;
-    const int64_t rds = (int64_t)((b + kCholNB - 1) / kCholNB) * kCholNB * kCholNB;
-    DevBuf S(D.pool.get(), nj * bb * sizeof(double2));
-    DevBuf T(D.pool.get(), blocked ? 0 : nj * bb * sizeof(double2));
-    DevBuf Rd(D.pool.get(), blocked ? (size_t)nj * rds * sizeof(double2) : 0);
+    DevBuf S(D.pool.get(), nj * bb * sizeof(double2)), T(D.pool.get(), nj * bb * sizeof(double2));
     DevBuf live(D.pool.get(), (size_t)nj * b * sizeof(int)), dmax(D.pool.get(), (size_t)nj * sizeof(double));
     DevBuf sz = upload_vec(D, std::vector<int>(nj, b));
     std::vector<int> hl((size_t)nj * b);
@@ -965,15 +961,8 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
             gemm_jobs(D, 'C', 'N', g);
             EventTimer ev;
             ev.start(D.stream);
-            if (blocked) {
-                launch_chol_blocked(D.stream, S.as<double2>(), sz.as<int>(), b, (int64_t)bb, kBlockDrop,
-                                    live.as<int>(), dmax.as<double>(), Rd.as<double2>(), rds, nj);
-            } else {
-                launch_chol_upper(D.stream, S.as<double2>(), sz.as<int>(), b, (int64_t)bb, kBlockDrop, live.as<int>(),
-                                  dmax.as<double>(), nj);
-                launch_tri_inv(D.stream, S.as<double2>(), T.as<double2>(), sz.as<int>(), live.as<int>(), b,
-                               (int64_t)bb, b, nj);
-            }
+            launch_chol_left(D.stream, S.as<double2>(), sz.as<int>(), b, b, (int64_t)bb, kBlockDrop, live.as<int>(),
+                             dmax.as<double>(), T.as<double2>(), nj);
             ev.stop(D.stream);
             D.blk.orth_ms += ev.ms();
             D.blk.orth_calls += 1;
@@ -986,22 +975,15 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
                 launch_refill_cols(D.stream, Y[q]->as<double2>(), n[q], b, live.as<int>() + (size_t)q * b,
                                    dmax.as<double>() + q, seed + 977u * (uint64_t)(pass * 3 + attempt + 1) + q);
         }
+        std::vector<DevBuf> Qn;
+        std::vector<GemmJob> g;
         for (int q = 0; q < nj; ++q) {
-            DevBuf Qn(D.pool.get(), (size_t)n[q] * b * sizeof(double2));
-            if (blocked) {
-                EventTimer ev;
-                ev.start(D.stream);
-                launch_trsm_rows(D.stream, Y[q]->as<double2>(), Qn.as<double2>(), n[q], b, S.as<double2>() + q * bb, b,
-                                 Rd.as<double2>() + (size_t)q * rds);
-                ev.stop(D.stream);
-                D.blk.orth_ms += ev.ms();
-            } else {
-                std::vector<GemmJob> g{GemmJob{Y[q]->as<double2>(), b, T.as<double2>() + q * bb, b, Qn.as<double2>(),
-                                               n[q], b, b}};
-                gemm_jobs(D, 'N', 'N', g);
-            }
-            *Y[q] = std::move(Qn);
+            Qn.emplace_back(D.pool.get(), (size_t)n[q] * b * sizeof(double2));
+            g.push_back(GemmJob{Y[q]->as<double2>(), b, T.as<double2>() + q * bb, b, Qn.back().as<double2>(), n[q], b,
+                                b});
         }
+        gemm_jobs(D, 'N', 'N', g);
+        for (int q = 0; q < nj; ++q) *Y[q] = std::move(Qn[q]);
     }
 }
 

@@ -98,30 +98,20 @@ void launch_trd_matvec(hipStream_t st, const double2* A, const double2* VW, doub
 // Block subspace route of the bond SVD.
 // p[e] = uniform [-1, 1)^2 complex value determined by (seed, e).
 void launch_rand_fill(hipStream_t st, double2* p, int64_t n, uint64_t seed);
-// Largest side of the Cholesky / triangular-inverse kernels; panel width and largest side of
-// the blocked Cholesky and its row-parallel triangular solve (16 x side complex in LDS).
+// Largest block of the route's Cholesky QR, and the panel width of its Cholesky.
 constexpr int kCholMax = 1024;
-constexpr int kCholNB = 16;
-constexpr int kCholBlockedMax = 512;
-// In place upper Cholesky factor of `batch` row-major Hermitian matrices (side sizes[b], ld,
-// stride); pivots <= tau x the largest diagonal drop their column (live[b ld + k] = 0);
-// dmax[b] (if not null) receives that largest diagonal.
-void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
-                       double* dmax, int batch);
-// The same factorisation blocked by 16-column panels (side <= kCholBlockedMax); Rd[b rd_stride +
-// p 256 + i 16 + j] receives the inverse of diagonal block p (zero for dropped indices).
-void launch_chol_blocked(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau,
-                         int* live, double* dmax, double2* Rd, int64_t rd_stride, int batch);
-// Q = Y R^{-1} (Y n x m row-major; R upper, ld ldr, and Rd from launch_chol_blocked); m <= 512.
-void launch_trsm_rows(hipStream_t st, const double2* Y, double2* Q, int n, int m, const double2* R, int ldr,
-                      const double2* Rd);
+constexpr int kCholLeftNB = 32;
+// Batched left-looking blocked Cholesky S = R^H R (upper, row-major, ld, batch stride) with the
+// rank-tolerant pivot drop (tau * the largest diagonal entry, into dmax), forming T = R^{-1}
+// (same layout; dropped columns give zero rows and columns) and the live flags.  S's tiles
+// right of each diagonal block receive R; its diagonal tiles keep their input.  One launch per
+// 32-column panel.
+void launch_chol_left(hipStream_t st, double2* S, const int* sizes, int nmax, int ld, int64_t stride, double tau,
+                      int* live, double* dmax, double2* T, int batch);
 // Columns c < ld of Y (n x ld, row-major) with live[c] == 0 refilled with seeded random values
 // whose squared column norm is about dmax[0].
 void launch_refill_cols(hipStream_t st, double2* Y, int n, int ld, const int* live, const double* dmax,
                         uint64_t seed);
-// T = R^{-1} over the live columns (dropped rows and columns zero); nmax = largest side.
-void launch_tri_inv(hipStream_t st, const double2* R, double2* T, const int* sizes, const int* live, int ld,
-                    int64_t stride, int nmax, int batch);
 // part[y kc + j] = sum over row chunk y of |G[r][j] - theta_j Z[r][j]|^2.
 void launch_resid_cols(hipStream_t st, const double2* G, int ldg, const double2* Z, int ldz, const double* theta,
                        int n, int kc, int rows_per, double* part);

@@ -1538,69 +1538,6 @@ __device__ inline double wave_max(double v) {
     return v;
 }
 
-// Upper Cholesky factor S = R^H R of row-major Hermitian matrices (the upper triangle is
-// read), one 1024-thread workgroup per matrix, right-looking, in place.  A pivot at or below
-// tau x the largest diagonal entry marks its column dependent: its row of R is zero and it
-// takes no part in the elimination, so R is the factor of S restricted to the live columns.
-// live[b ld + k] = 1 for a live column.  n = sizes[b] <= kCholMax.
-__global__ __launch_bounds__(1024) void chol_upper_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
-                                                          int ld, int64_t stride, double tau,
-                                                          int* __restrict__ live, double* __restrict__ dmax_out) {
-    const int b = blockIdx.x, n = sizes[b];
-    S += (int64_t)b * stride;
-    live += (int64_t)b * ld;
-    __shared__ double2 rk[kCholMax];
-    __shared__ double red[16];
-    __shared__ double dmax_s, rinv_s;
-    __shared__ int live_s;
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    double m = 0.0;
-    for (int i = tid; i < n; i += 1024) m = fmax(m, S[(int64_t)i * ld + i].x);
-    m = wave_max(m);
-    if (lane == 0) red[wv] = m;
-    __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < 16; ++w) t = fmax(t, red[w]);
-        dmax_s = t;
-        if (dmax_out) dmax_out[b] = t;
-    }
-    __syncthreads();
-    const double thr = tau * dmax_s;
-    for (int k = 0; k < n; ++k) {
-        if (tid == 0) {
-            const double piv = S[(int64_t)k * ld + k].x;
-            const int lv = (piv > thr && piv > 0.0) ? 1 : 0;  // NaN is not live
-            live_s = lv;
-            rinv_s = lv ? 1.0 / sqrt(piv) : 0.0;
-            live[k] = lv;
-        }
-        __syncthreads();
-        const double ri = rinv_s;
-        const int lv = live_s;
-        for (int j = k + tid; j < n; j += 1024) {
-            double2 v = S[(int64_t)k * ld + j];
-            v = (j == k) ? make_double2(lv ? 1.0 / ri : 0.0, 0.0) : make_double2(v.x * ri, v.y * ri);
-            rk[j] = v;
-            S[(int64_t)k * ld + j] = v;
-        }
-        __syncthreads();
-        if (lv) {  // S_ij -= conj(R_ki) R_kj, k < i <= j
-            for (int i = k + 1 + wv; i < n; i += 16) {
-                const double2 a = rk[i];
-                for (int j = i + lane; j < n; j += 64) {
-                    const double2 c = rk[j];
-                    double2 x = S[(int64_t)i * ld + j];
-                    x.x -= a.x * c.x + a.y * c.y;
-                    x.y -= a.x * c.y - a.y * c.x;
-                    S[(int64_t)i * ld + j] = x;
-                }
-            }
-        }
-        __syncthreads();
-    }
-}
-
 __device__ inline double2 cfma_conj(double2 acc, double2 a, double2 b) {  // acc + conj(a) b
     return make_double2(acc.x + a.x * b.x + a.y * b.y, acc.y + a.x * b.y - a.y * b.x);
 }
@@ -1608,249 +1545,203 @@ __device__ inline double2 cfma(double2 acc, double2 a, double2 b) {  // acc + a 
     return make_double2(acc.x + a.x * b.x - a.y * b.y, acc.y + a.x * b.y + a.y * b.x);
 }
 
-// The same factorisation blocked by panels of kCholNB = 16 columns, for sides up to
-// kCholBlockedMax, one 512-thread workgroup per matrix: wave 0 factors the diagonal block in LDS (the same pivot drops) and
-// inverts it over its live indices; every thread then forms one column of the panel rows
-// R_{p,j} = R_pp^{-H} S_{p,j} (kept in LDS), and the trailing triangle takes the rank-16
-// update S_ij -= sum_l conj(R_li) R_lj with one read-modify-write per element per panel.  Rd
-// receives each diagonal block's inverse (16 x 16 row-major, zero rows and columns for dropped
-// indices) for trsm_rows_kernel.
-constexpr int kCholBlockedThreads = 512;
-__global__ __launch_bounds__(kCholBlockedThreads) void chol_blocked_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
-                                                            int ld, int64_t stride, double tau,
-                                                            int* __restrict__ live, double* __restrict__ dmax_out,
-                                                            double2* __restrict__ Rd, int64_t rd_stride) {
-    constexpr int NB = kCholNB, NT = kCholBlockedThreads, NW = NT / 64;
+// Largest real diagonal entry of each batched matrix (the Cholesky drop threshold's scale),
+// read before any panel step overwrites S.  One workgroup per matrix.
+__global__ __launch_bounds__(256) void diag_max_kernel(const double2* __restrict__ S, const int* __restrict__ sizes,
+                                                       int ld, int64_t stride, double* __restrict__ dmax) {
     const int b = blockIdx.x, n = sizes[b];
     S += (int64_t)b * stride;
-    live += (int64_t)b * ld;
-    Rd += (int64_t)b * rd_stride;
-    __shared__ double2 Dg[NB][NB + 1], Di[NB][NB + 1];
-    __shared__ double2 panel[NB][kCholBlockedMax];
-    __shared__ double red[NW];
-    __shared__ double thr_s;
-    __shared__ int lv_s[NB];
-    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const double2 zero2 = make_double2(0.0, 0.0);
+    __shared__ double red[4];
     double mx = 0.0;
-    for (int i = tid; i < n; i += NT) mx = fmax(mx, S[(int64_t)i * ld + i].x);
+    for (int i = threadIdx.x; i < n; i += 256) mx = fmax(mx, S[(int64_t)i * ld + i].x);
     mx = wave_max(mx);
-    if (lane == 0) red[wv] = mx;
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
     __syncthreads();
-    if (tid == 0) {
-        double t = 0.0;
-        for (int w = 0; w < NW; ++w) t = fmax(t, red[w]);
-        thr_s = tau * t;
-        if (dmax_out) dmax_out[b] = t;
-    }
-    __syncthreads();
-    const double thr = thr_s;
-    for (int p = 0; p < n; p += NB) {
-        const int w = min(NB, n - p);
-        if (wv == 0) {  // diagonal block: factor and invert in LDS; one wave's LDS ops stay in order
-            for (int e = lane; e < NB * NB; e += 64) {
-                Dg[e / NB][e % NB] = zero2;
-                Di[e / NB][e % NB] = zero2;
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int e = lane; e < w * w; e += 64) {  // (a select of a load here takes a stack slot)
-                const int i = e / w, j = e % w;
-                if (j >= i) Dg[i][j] = S[(int64_t)(p + i) * ld + p + j];
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int k = 0; k < w; ++k) {
-                const double piv = Dg[k][k].x;
-                const bool lk = piv > thr && piv > 0.0;  // NaN is not live
-                const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
-                __builtin_amdgcn_wave_barrier();
-                if (lane > k && lane < w) Dg[k][lane] = make_double2(Dg[k][lane].x * ri, Dg[k][lane].y * ri);
-                if (lane == k) {
-                    Dg[k][k] = make_double2(lk ? sqrt(piv) : 0.0, 0.0);
-                    lv_s[k] = lk ? 1 : 0;
-                }
-                __builtin_amdgcn_wave_barrier();
-                if (lk)
-                    for (int e = lane; e < NB * NB; e += 64) {
-                        const int i = e / NB, j = e % NB;
-                        if (i > k && j >= i && j < w) {
-                            const double2 a = Dg[k][i], c = Dg[k][j];
-                            Dg[i][j] = make_double2(Dg[i][j].x - (a.x * c.x + a.y * c.y),
-                                                    Dg[i][j].y - (a.x * c.y - a.y * c.x));
-                        }
-                    }
-                __builtin_amdgcn_wave_barrier();
-            }
-            if (lane < w && lv_s[lane]) {  // column `lane` of the inverse, by back substitution
-                const int c = lane;
-                Di[c][c] = make_double2(1.0 / Dg[c][c].x, 0.0);
-                for (int i = c - 1; i >= 0; --i) {
-                    double2 acc = zero2;
-                    if (lv_s[i]) {
-                        for (int k = i + 1; k <= c; ++k) acc = cfma(acc, Dg[i][k], Di[k][c]);
-                        acc = make_double2(-acc.x / Dg[i][i].x, -acc.y / Dg[i][i].x);
-                    }
-                    Di[i][c] = acc;
-                }
-            }
-            __builtin_amdgcn_wave_barrier();
-            for (int e = lane; e < NB * NB; e += 64) {
-                const int i = e / NB, j = e % NB;
-                if (i < w && j < w && j >= i) S[(int64_t)(p + i) * ld + p + j] = Dg[i][j];
-                Rd[(int64_t)(p / NB) * NB * NB + e] = Di[i][j];
-            }
-            if (lane < w) live[p + lane] = lv_s[lane];
-        }
-        __syncthreads();
-        const int t0 = p + w, mt = n - t0;
-        for (int c = tid; c < mt; c += NT) {  // panel rows, one column each: y = Rinv_pp^H x
-            double2 y[NB];
-#pragma unroll
-            for (int l = 0; l < NB; ++l) y[l] = zero2;
-            // m stays a loop (the compiler would otherwise hoist all of Di into registers) and l
-            // runs over every slot with a predicate, so y stays in registers
-#pragma unroll 1
-            for (int m = 0; m < w; ++m) {
-                const double2 xm = S[(int64_t)(p + m) * ld + t0 + c];
-#pragma unroll
-                for (int l = 0; l < NB; ++l)
-                    if (l >= m) y[l] = cfma_conj(y[l], Di[m][l], xm);
-            }
-#pragma unroll
-            for (int l = 0; l < NB; ++l)
-                if (l < w) {
-                    S[(int64_t)(p + l) * ld + t0 + c] = y[l];
-                    panel[l][c] = y[l];
-                }
-        }
-        __syncthreads();
-        for (int r = wv; r < mt; r += NW) {  // trailing triangle: row t0 + r per wave
-            double2 a[NB];
-#pragma unroll
-            for (int l = 0; l < NB; ++l) a[l] = l < w ? panel[l][r] : zero2;
-            double2* row = S + (int64_t)(t0 + r) * ld + t0;
-            for (int c = r + lane; c < mt; c += 64) {
-                double2 acc = zero2;
-#pragma unroll
-                for (int l = 0; l < NB; ++l) acc = cfma_conj(acc, a[l], panel[l][c]);
-                const double2 v = row[c];
-                row[c] = make_double2(v.x - acc.x, v.y - acc.y);
-            }
-        }
-        __syncthreads();
-    }
+    if (threadIdx.x == 0) dmax[b] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// Q = Y R^{-1} (Y n x m row-major, R the upper factor of chol_blocked_kernel in S with ld ldr,
-// Rd its diagonal blocks' inverses), rows in parallel: 16 rows per workgroup kept in LDS, and
-// per panel p of 16 columns Q_p = (Y_p - Q_{<p} R_{<p,p}) Rinv_pp.  grid ceil(n / 16).
-__global__ __launch_bounds__(256) void trsm_rows_kernel(const double2* __restrict__ Y, double2* __restrict__ Q,
-                                                        int n, int m, const double2* __restrict__ R, int ldr,
-                                                        const double2* __restrict__ Rd) {
-    constexpr int NB = kCholNB;
-    __shared__ double2 q[NB][kCholBlockedMax];
-    __shared__ double2 t[NB][NB + 1];
-    const int tr = threadIdx.x / NB, tc = threadIdx.x % NB;
-    const int row = blockIdx.x * NB + tr;
-    const double2 zero2 = make_double2(0.0, 0.0);
-    for (int p = 0; p < m; p += NB) {
-        const int w = min(NB, m - p);
-        double2 u = zero2;
-        if (row < n && tc < w) {
-            u = Y[(int64_t)row * m + p + tc];
-            double2 acc = zero2;
-            for (int k = 0; k < p; ++k) acc = cfma(acc, q[tr][k], R[(int64_t)k * ldr + p + tc]);
-            u = make_double2(u.x - acc.x, u.y - acc.y);
-        }
-        t[tr][tc] = u;
-        __syncthreads();
-        double2 x = zero2;
-        if (tc < w) {
-            const double2* rd = Rd + (int64_t)(p / NB) * NB * NB;
-            for (int l = 0; l <= tc; ++l) x = cfma(x, t[tr][l], rd[l * NB + tc]);
-            q[tr][p + tc] = x;
-            if (row < n) Q[(int64_t)row * m + p + tc] = x;
-        }
-        __syncthreads();
-    }
-}
-
-void launch_chol_blocked(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau,
-                         int* live, double* dmax, double2* Rd, int64_t rd_stride, int batch) {
-    KV_REQUIRE(ld <= kCholBlockedMax && batch >= 1, ERR_INTERNAL, "chol_blocked: side above kCholBlockedMax");
-    KV_REQUIRE(rd_stride >= (int64_t)((ld + kCholNB - 1) / kCholNB) * kCholNB * kCholNB, ERR_INTERNAL,
-               "chol_blocked: diagonal-block buffer too small");
-    hipLaunchKernelGGL(chol_blocked_kernel, dim3(batch), dim3(kCholBlockedThreads), 0, st, S, sizes, ld, stride, tau,
-                       live, dmax, Rd, rd_stride);
-    KV_HIP(hipGetLastError());
-}
-
-void launch_trsm_rows(hipStream_t st, const double2* Y, double2* Q, int n, int m, const double2* R, int ldr,
-                      const double2* Rd) {
-    KV_REQUIRE(m <= kCholBlockedMax && m <= ldr, ERR_INTERNAL, "trsm_rows: more columns than the LDS row block holds");
-    if (n <= 0 || m <= 0) return;
-    hipLaunchKernelGGL(trsm_rows_kernel, dim3((n + kCholNB - 1) / kCholNB), dim3(kCholNB * kCholNB), 0, st, Y, Q, n, m,
-                       R, ldr, Rd);
-    KV_HIP(hipGetLastError());
-}
-
-// T = R^{-1} over the live columns of an upper factor from chol_upper_kernel (row-major, rows
-// and columns of dropped indices zero).  One wave per column c: back substitution
-// T_ic = -(sum_{i<k<=c} R_ik T_kc) / R_ii from i = c down, with column c kept in LDS and the
-// next row of R loaded while the current one is reduced.  grid (ceil(nmax / 4), batch).
-__global__ __launch_bounds__(256) void tri_inv_kernel(const double2* __restrict__ R, double2* __restrict__ T,
-                                                      const int* __restrict__ sizes,
-                                                      const int* __restrict__ live, int ld, int64_t stride) {
+// Panel step k of a left-looking blocked Cholesky S = R^H R (upper R, row-major, panels of
+// NB = kCholLeftNB columns) that also forms T = R^{-1}.  Every workgroup first forms the
+// panel's diagonal block C_kk = S_kk - R_{<k,k}^H R_{<k,k} and factors it in LDS, right-looking
+// with a rank-tolerant pivot rule: a pivot at or below tau * dmax marks its column dependent,
+// its row of R and of T are zero and it takes no part in the elimination (R is the factor of
+// S restricted to the live columns).  Every workgroup does the same arithmetic.  Then
+//   workgroup j < (panels from k on): column panel c = k + j; for j > 0
+//     R_kc = R_kk^{-H} (S_kc - R_{<k,k}^H R_{<k,c}) into S's tile (k, c) and a zero tile
+//     (c, k) of T; for j = 0 the inverse diagonal block into T's tile (k, k) and the live flags;
+//   workgroup j < k: T's tile (j, k) = -(T_{j,j..k-1} R_{j..k-1,k}) R_kk^{-1}.
+// Tiles of S written in step k (rows of panel k right of its diagonal block) are read by no
+// other workgroup of the step; the diagonal tile of S is left as it was (only T is used
+// afterwards).  grid (ceil(nmax / NB), batch), 256 threads; steps k = 0, 1, ... in order.
+__global__ __launch_bounds__(256) void chol_left_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
+                                                        int ld, int64_t stride, double tau,
+                                                        int* __restrict__ live, const double* __restrict__ dmax,
+                                                        double2* __restrict__ T, int k) {
+    constexpr int NB = kCholLeftNB;
     const int b = blockIdx.y, n = sizes[b];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int c = blockIdx.x * 4 + wv;
-    R += (int64_t)b * stride;
+    const int p0 = k * NB;
+    if (p0 >= n) return;  // uniform over the workgroup
+    const int w = min(NB, n - p0);
+    const int j = blockIdx.x;
+    const int npk = (n - p0 + NB - 1) / NB;
+    const bool chol_role = j < npk;
+    const bool inv_role = j < k;
+    if (!chol_role && !inv_role) return;  // uniform over the workgroup
+    S += (int64_t)b * stride;
     T += (int64_t)b * stride;
     live += (int64_t)b * ld;
-    __shared__ double2 xs[4][kCholMax];
-    if (c >= n) return;  // c is uniform over the wave, and nothing below synchronises the workgroup
-    double2* x = xs[wv];
-    const bool lc = live[c] != 0;
-    constexpr int U = kCholMax / 64;
-    double2 rn[U];  // row i of R, entries k = i + 1 + lane + 64u
-    auto load_row = [&](int i) {
+    const double thr = tau * dmax[b];
+    __shared__ double2 At[NB][NB + 1], Bt[NB][NB + 1], Ct[NB][NB + 1];
+    __shared__ double2 Dg[NB][NB + 1], Di[NB][NB + 1];
+    __shared__ int lv_s[NB];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r = tid >> 3, c4 = (tid & 7) * 4;  // this thread's 4 outputs of a 32 x 32 tile
+    const double2 zero2 = make_double2(0.0, 0.0);
+    const int c0 = p0 + j * NB;                       // chol role: column panel
+    const int wc = chol_role ? min(NB, n - c0) : 0;
+    const bool off = chol_role && j > 0;
+    // C_kk and C_kc: the original tiles minus the rows of R above panel k
+    double2 ak[4], ac[4];
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            const int k = i + 1 + lane + 64 * u;
-            rn[u] = (k <= c) ? R[(int64_t)i * ld + k] : make_double2(0.0, 0.0);
+    for (int q = 0; q < 4; ++q) ak[q] = ac[q] = zero2;
+    for (int i0 = 0; i0 < p0; i0 += NB) {
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int ii = e / NB, cc = e % NB;
+            double2 va = zero2, vb = zero2;
+            if (cc < w) va = S[(int64_t)(i0 + ii) * ld + p0 + cc];
+            if (off && cc < wc) vb = S[(int64_t)(i0 + ii) * ld + c0 + cc];
+            At[ii][cc] = va;
+            Bt[ii][cc] = vb;
         }
-    };
-    if (lc) load_row(c);
-    for (int i = c; i >= 0; --i) {
-        double2 val = make_double2(0.0, 0.0);
-        if (lc && live[i]) {
-            double2 acc = make_double2(0.0, 0.0);
+        __syncthreads();
+#pragma unroll 4
+        for (int ii = 0; ii < NB; ++ii) {
+            const double2 a = At[ii][r];
 #pragma unroll
-            for (int u = 0; u < U; ++u) {
-                const int k = i + 1 + lane + 64 * u;
-                if (k <= c) {
-                    const double2 r = rn[u], t = x[k];
-                    acc.x += r.x * t.x - r.y * t.y;
-                    acc.y += r.x * t.y + r.y * t.x;
-                }
+            for (int q = 0; q < 4; ++q) {
+                ak[q] = cfma_conj(ak[q], a, At[ii][c4 + q]);
+                if (off) ac[q] = cfma_conj(ac[q], a, Bt[ii][c4 + q]);
             }
-            const double rii = R[(int64_t)i * ld + i].x;
-            if (i > 0) load_row(i - 1);
-            acc = wave_sum2(acc);
-            val = (i == c) ? make_double2(1.0 / rii, 0.0) : make_double2(-acc.x / rii, -acc.y / rii);
-        } else if (lc && i > 0) {
-            load_row(i - 1);
         }
-        x[i] = val;  // every lane holds the same value (butterfly sum); the wave's LDS ops stay in order
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();
     }
-    for (int i = lane; i < n; i += 64) T[(int64_t)i * ld + c] = (i <= c) ? x[i] : make_double2(0.0, 0.0);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int c = c4 + q;
+        Dg[r][c] = zero2;
+        Di[r][c] = zero2;
+        Ct[r][c] = zero2;
+        if (r < w && c < w && c >= r) {
+            const double2 sv = S[(int64_t)(p0 + r) * ld + p0 + c];
+            Dg[r][c] = make_double2(sv.x - ak[q].x, sv.y - ak[q].y);
+        }
+        if (off && r < w && c < wc) {
+            const double2 sv = S[(int64_t)(p0 + r) * ld + c0 + c];
+            Ct[r][c] = make_double2(sv.x - ac[q].x, sv.y - ac[q].y);
+        }
+    }
+    __syncthreads();
+    if (wv == 0) {  // factor and invert the diagonal block; one wave's LDS ops stay in order
+        for (int kk = 0; kk < w; ++kk) {
+            const double piv = Dg[kk][kk].x;
+            const bool lk = piv > thr && piv > 0.0;  // NaN is not live
+            const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
+            __builtin_amdgcn_wave_barrier();
+            if (lane > kk && lane < w) Dg[kk][lane] = make_double2(Dg[kk][lane].x * ri, Dg[kk][lane].y * ri);
+            if (lane == kk) {
+                Dg[kk][kk] = make_double2(lk ? sqrt(piv) : 0.0, 0.0);
+                lv_s[kk] = lk ? 1 : 0;
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lk)
+                for (int e = lane; e < NB * NB; e += 64) {
+                    const int i = e / NB, jj = e % NB;
+                    if (i > kk && jj >= i && jj < w) {
+                        const double2 a = Dg[kk][i], c = Dg[kk][jj];
+                        Dg[i][jj] = make_double2(Dg[i][jj].x - (a.x * c.x + a.y * c.y),
+                                                 Dg[i][jj].y - (a.x * c.y - a.y * c.x));
+                    }
+                }
+            __builtin_amdgcn_wave_barrier();
+        }
+        if (lane < w && lv_s[lane]) {  // column `lane` of the inverse, by back substitution
+            const int c = lane;
+            Di[c][c] = make_double2(1.0 / Dg[c][c].x, 0.0);
+            for (int i = c - 1; i >= 0; --i) {
+                double2 acc = zero2;
+                if (lv_s[i]) {
+                    for (int kk = i + 1; kk <= c; ++kk) acc = cfma(acc, Dg[i][kk], Di[kk][c]);
+                    acc = make_double2(-acc.x / Dg[i][i].x, -acc.y / Dg[i][i].x);
+                }
+                Di[i][c] = acc;
+            }
+        }
+    }
+    __syncthreads();
+    if (chol_role && j == 0) {
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int i = e / NB, jj = e % NB;
+            if (i < w && jj < w) T[(int64_t)(p0 + i) * ld + p0 + jj] = Di[i][jj];
+        }
+        if (tid < w) live[p0 + tid] = lv_s[tid];
+    }
+    if (off) {  // R_kc = Di^H C_kc, and the zero tile of T below the diagonal
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = c4 + q;
+            double2 y = zero2;
+            for (int m = 0; m <= r; ++m) y = cfma_conj(y, Di[m][r], Ct[m][c]);
+            if (r < w && c < wc) S[(int64_t)(p0 + r) * ld + c0 + c] = y;
+            if (c < w && r < wc) T[(int64_t)(c0 + r) * ld + p0 + c] = zero2;
+        }
+    }
+    if (inv_role) {  // T_jk = -(T_{j, j..k-1} R_{j..k-1, k}) Di
+        const int R0 = j * NB;
+        double2 x[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) x[q] = zero2;
+        __syncthreads();  // At, Bt are reused
+        for (int i0 = R0; i0 < p0; i0 += NB) {
+            for (int e = tid; e < NB * NB; e += 256) {
+                const int ii = e / NB, cc = e % NB;
+                double2 vb = zero2;
+                if (cc < w) vb = S[(int64_t)(i0 + ii) * ld + p0 + cc];  // R rows i0.., panel k
+                At[ii][cc] = T[(int64_t)(R0 + ii) * ld + i0 + cc];    // T rows of block j
+                Bt[ii][cc] = vb;
+            }
+            __syncthreads();
+#pragma unroll 4
+            for (int ii = 0; ii < NB; ++ii) {
+                const double2 a = At[r][ii];
+#pragma unroll
+                for (int q = 0; q < 4; ++q) x[q] = cfma(x[q], a, Bt[ii][c4 + q]);
+            }
+            __syncthreads();
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) Ct[r][c4 + q] = x[q];
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const int c = c4 + q;
+            double2 y = zero2;
+            for (int m = 0; m <= c; ++m) y = cfma(y, Ct[r][m], Di[m][c]);
+            if (c < w) T[(int64_t)(R0 + r) * ld + p0 + c] = make_double2(-y.x, -y.y);
+        }
+    }
 }
 
-void launch_chol_upper(hipStream_t st, double2* S, const int* sizes, int ld, int64_t stride, double tau, int* live,
-                       double* dmax, int batch) {
-    KV_REQUIRE(ld <= kCholMax && batch >= 1, ERR_INTERNAL, "chol_upper: side above kCholMax");
-    hipLaunchKernelGGL(chol_upper_kernel, dim3(batch), dim3(1024), 0, st, S, sizes, ld, stride, tau, live, dmax);
+void launch_chol_left(hipStream_t st, double2* S, const int* sizes, int nmax, int ld, int64_t stride, double tau,
+                      int* live, double* dmax, double2* T, int batch) {
+    KV_REQUIRE(nmax <= ld && batch >= 1, ERR_INTERNAL, "chol_left: side above the leading dimension");
+    hipLaunchKernelGGL(diag_max_kernel, dim3(batch), dim3(256), 0, st, S, sizes, ld, stride, dmax);
     KV_HIP(hipGetLastError());
+    const int np = (nmax + kCholLeftNB - 1) / kCholLeftNB;
+    for (int k = 0; k < np; ++k) {
+        hipLaunchKernelGGL(chol_left_kernel, dim3(np, batch), dim3(256), 0, st, S, sizes, ld, stride, tau, live,
+                           dmax, T, k);
+        KV_HIP(hipGetLastError());
+    }
 }
 
 // Column c of Y (n x ld, row-major) for every c < ld with live[c] == 0: seeded uniform
@@ -1874,14 +1765,6 @@ void launch_refill_cols(hipStream_t st, double2* Y, int n, int ld, const int* li
     if (total <= 0) return;
     const int blocks = (int)std::min<int64_t>((total + 255) / 256, 4096);
     hipLaunchKernelGGL(refill_cols_kernel, dim3(blocks), dim3(256), 0, st, Y, n, ld, live, dmax, seed);
-    KV_HIP(hipGetLastError());
-}
-
-void launch_tri_inv(hipStream_t st, const double2* R, double2* T, const int* sizes, const int* live, int ld,
-                    int64_t stride, int nmax, int batch) {
-    KV_REQUIRE(ld <= kCholMax && nmax <= ld && batch >= 1, ERR_INTERNAL, "tri_inv: side above kCholMax");
-    if (nmax <= 0) return;
-    hipLaunchKernelGGL(tri_inv_kernel, dim3((nmax + 3) / 4, batch), dim3(256), 0, st, R, T, sizes, live, ld, stride);
     KV_HIP(hipGetLastError());
 }
 

@@ -3,12 +3,13 @@
 The device route (randomised range finder with power steps and Rayleigh-Ritz) has no
 counterpart in the reference, which calls LAPACK's full SVD per sector.  This restates its
 steps in numpy — Cholesky QR with pivots at or below 1e-12 x the largest dropped
-(right-looking, the order chol_upper_kernel eliminates in; chol_blocked_kernel's 16-column
-panels with the row-parallel solve trsm_rows_kernel give the same factor and basis), the
-inverse of the factor by back substitution (tri_inv_kernel), the compaction of live columns, the power steps
+(chol_upper: the unblocked right-looking elimination; chol_left: chol_left_kernel's
+left-looking 32-column panels, which also form the inverse factor T, checked against the
+unblocked factor and its inverse by back substitution), the power steps
 Q -> orth(A^H Q) -> orth(A P), the Ritz pairs, their residuals and the ceiling on the values
-beyond the certified ones, and the check that settles a cut — and checks them against LAPACK (numpy.linalg.svd):
-orthonormality of the Cholesky QR bases, the certified singular values, the isometry and
+beyond the certified ones, and the check that settles a cut — and checks them against LAPACK
+(numpy.linalg.svd): orthonormality of the Cholesky QR bases, the certified singular values,
+the isometry and
 the truncated product.  Planted slips (no second Cholesky QR pass, no power step) must fail.
 """
 import numpy as np
@@ -53,19 +54,21 @@ def tri_inv(R, live):
     return T
 
 
-def chol_blocked(S, nb=16, tau=DROP):
-    """chol_blocked_kernel's order: per panel of nb columns the diagonal block's factor (with
-    the same pivot drops) and inverse, the panel rows R_pp^{-H} S_pj, the rank-nb trailing
-    update.  Returns (R, live, inverses of the diagonal blocks)."""
+def chol_left(S, nb=32, tau=DROP):
+    """chol_left_kernel's order: per panel k of nb columns C = S_{k,k:} - R_{<k,k}^H R_{<k,k:};
+    its diagonal block factored right-looking with the same pivot drops and inverted (Di);
+    R_{k,c} = Di^H C_{k,c} right of it; T_{k,k} = Di and T_{<k,k} = -(T_{<k,<k} R_{<k,k}) Di.
+    Returns (R, live, T)."""
     S = np.array(S, dtype=complex)
     n = S.shape[0]
     thr = tau * max(np.real(np.diag(S)).max(), 0.0)
     R = np.zeros_like(S)
+    T = np.zeros_like(S)
     live = np.zeros(n, dtype=bool)
-    Rd = []
     for p in range(0, n, nb):
         w = min(nb, n - p)
-        Dg = np.triu(S[p:p + w, p:p + w])
+        C = S[p:p + w, p:] - R[:p, p:p + w].conj().T @ R[:p, p:]
+        Dg = np.triu(C[:, :w])
         lv = np.zeros(w, dtype=bool)
         for k in range(w):
             piv = Dg[k, k].real
@@ -77,7 +80,7 @@ def chol_blocked(S, nb=16, tau=DROP):
             if lk:
                 for i in range(k + 1, w):
                     Dg[i, i:] -= np.conj(Dg[k, i]) * Dg[k, i:]
-        Di = np.zeros((nb, nb), dtype=complex)
+        Di = np.zeros((w, w), dtype=complex)
         for c in range(w):
             if not lv[c]:
                 continue
@@ -85,25 +88,12 @@ def chol_blocked(S, nb=16, tau=DROP):
             for i in range(c - 1, -1, -1):
                 if lv[i]:
                     Di[i, c] = -(Dg[i, i + 1:c + 1] @ Di[i + 1:c + 1, c]) / Dg[i, i].real
-        Rd.append(Di)
         live[p:p + w] = lv
         R[p:p + w, p:p + w] = np.triu(Dg)
-        t0 = p + w
-        panel = Di[:w, :w].conj().T @ S[p:p + w, t0:]
-        R[p:p + w, t0:] = panel
-        S[t0:, t0:] -= panel.conj().T @ panel
-    return R, live, Rd
-
-
-def trsm_rows(Y, R, Rd, nb=16):
-    """trsm_rows_kernel's order: per panel Q_p = (Y_p - Q_{<p} R_{<p,p}) Rinv_pp."""
-    n, m = Y.shape
-    Q = np.zeros_like(Y)
-    for p in range(0, m, nb):
-        w = min(nb, m - p)
-        u = Y[:, p:p + w] - Q[:, :p] @ R[:p, p:p + w]
-        Q[:, p:p + w] = u @ Rd[p // nb][:w, :w]
-    return Q
+        R[p:p + w, p + w:] = Di.conj().T @ C[:, w:]
+        T[p:p + w, p:p + w] = Di
+        T[:p, p:p + w] = -(T[:p, :p] @ R[:p, p:p + w]) @ Di
+    return R, live, T
 
 
 def block_orth(Y, passes=2, refill=False, seed=99):
@@ -113,7 +103,7 @@ def block_orth(Y, passes=2, refill=False, seed=99):
     for _ in range(passes):
         for attempt in range(3):
             S = Y.conj().T @ Y
-            R, live = chol_upper(S)
+            R, live, T = chol_left(S)
             if live.all() or not refill or attempt == 2:
                 break
             scale = np.sqrt(1.5 * np.real(np.diag(S)).max() / Y.shape[0])
@@ -121,7 +111,7 @@ def block_orth(Y, passes=2, refill=False, seed=99):
             Y = Y.copy()
             Y[:, dead] = scale * (rng.uniform(-1, 1, (Y.shape[0], dead.sum()))
                                   + 1j * rng.uniform(-1, 1, (Y.shape[0], dead.sum())))
-        Y = Y @ tri_inv(R, live)
+        Y = Y @ T
     return Y
 
 
@@ -195,25 +185,27 @@ def test_cholesky_qr_with_dropped_columns_matches_lapack_qr():
     np.testing.assert_allclose(Qr.conj().T @ Qr, np.eye(40), atol=1e-13)
 
 
-def test_blocked_cholesky_and_row_solve_match_the_unblocked_factor():
-    """The panel-blocked factorisation and the row-parallel solve give the unblocked factor
-    R and Q = Y R^{-1} over the live columns, with dependent columns inside a panel (7, 20)
-    and on a panel edge (31), and a side (45) that is not a multiple of the panel."""
+def test_left_looking_cholesky_and_its_inverse_match_the_unblocked_factor():
+    """The left-looking panel factorisation gives the unblocked factor R and T = R^{-1} over
+    the live columns, with dependent columns inside a panel (7, 40), on a panel edge (63) and
+    in the last, partial panel (70), on a side (75) that is not a multiple of the panel; Y T
+    has orthonormal live columns and zero dropped ones."""
     rng = np.random.default_rng(13)
-    Y = rng.standard_normal((200, 45)) + 1j * rng.standard_normal((200, 45))
+    Y = rng.standard_normal((300, 75)) + 1j * rng.standard_normal((300, 75))
     Y[:, 7] = Y[:, 3] - 2j * Y[:, 5]
-    Y[:, 20] = 0.5 * Y[:, 19]
-    Y[:, 31] = Y[:, 0] + Y[:, 30]
+    Y[:, 40] = 0.5 * Y[:, 39]
+    Y[:, 63] = Y[:, 0] + Y[:, 62]
+    Y[:, 70] = 1j * Y[:, 65]
     S = Y.conj().T @ Y
     R1, live1 = chol_upper(S)
-    R2, live2, Rd = chol_blocked(S)
-    assert np.array_equal(live1, live2) and live1.sum() == 42
+    R2, live2, T = chol_left(S)
+    assert np.array_equal(live1, live2) and live1.sum() == 71
     np.testing.assert_allclose(R2, R1, atol=1e-12 * np.abs(R1).max())
-    Q = trsm_rows(Y, R2, Rd)
-    np.testing.assert_allclose(Q, Y @ tri_inv(R1, live1), atol=1e-12)
+    np.testing.assert_allclose(T, tri_inv(R1, live1), atol=1e-12 * np.abs(T).max())
+    Q = Y @ T
     assert np.abs(Q[:, ~live1]).max() == 0.0
     Ql = Q[:, live1]
-    np.testing.assert_allclose(Ql.conj().T @ Ql, np.eye(42), atol=1e-12)
+    np.testing.assert_allclose(Ql.conj().T @ Ql, np.eye(71), atol=1e-12)
 
 
 def test_one_cholesky_qr_pass_is_not_enough_on_ill_conditioned_blocks():
