@@ -453,7 +453,13 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
         } else {
             trd_ev.start(D.stream);
-            hetrd_lower(D, sol, Gb.as<double2>(), N, nb, W.as<double>(), E.as<double>(), taub.as<double2>());
+            if (N <= kTrdSmallMax) {  // one workgroup per matrix, no per-column launches
+                launch_herm_fill(D.stream, Gb.as<double2>(), N, nb);
+                launch_trd_small(D.stream, Gb.as<double2>(), N, W.as<double>(), E.as<double>(), taub.as<double2>(),
+                                 nb);
+            } else {
+                hetrd_lower(D, sol, Gb.as<double2>(), N, nb, W.as<double>(), E.as<double>(), taub.as<double2>());
+            }
             trd_ev.stop(D.stream);
             trd_timed = true;
         }
@@ -1072,7 +1078,9 @@ inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool i
     {
         EventTimer ev;
         ev.start(D.stream);
-        gram_solve(D, sol, hp, mode, roc_trd);
+        // the split route (one-workgroup tridiagonalisation, dstedc, back-transformation):
+        // rocSOLVER's zheevd spends most of its time reducing a 256-wide matrix column by column
+        gram_solve(D, sol, hp, mode == EIG_HEEVD ? EIG_HEEVD : EIG_SPLIT, roc_trd);
         for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
         ev.stop(D.stream);
         D.blk.eig_ms += ev.ms();

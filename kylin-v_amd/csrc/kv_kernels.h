@@ -88,6 +88,10 @@ void launch_real_cols(hipStream_t st, const double* C, int ldc, int c0, int k, i
 // step; every step leaves it zero).
 constexpr int kTrdPanel = 32;
 void launch_herm_fill(hipStream_t st, double2* A, int n, int batch);
+// The same reduction (D, E, tau, stored reflectors) for sides up to kTrdSmallMax in one
+// workgroup per matrix; A must hold both triangles (launch_herm_fill).
+constexpr int kTrdSmallMax = 256;
+void launch_trd_small(hipStream_t st, double2* A, int n, double* D, double* E, double2* tau, int batch);
 int trd_column_blocks(int n, int j);
 void launch_trd_column(hipStream_t st, double2* A, double2* VW, const double2* y, const double2* q, double* D,
                        double* E, double2* tau, double2* part, unsigned* count, int n, int j, int i, int nb,

@@ -1476,6 +1476,194 @@ __global__ __launch_bounds__(256) void trd_matvec_kernel(const double2* __restri
     }
 }
 
+// Householder tridiagonalisation in zhetrd('L')'s form of Hermitian matrices up to
+// kTrdSmallMax wide, one 512-thread workgroup per matrix (column-major, both triangles
+// valid, as herm_fill leaves them).  Unblocked, with each step's rank-2 update fused into the
+// next step's matrix-vector product: after column j's reflector v_j and y_j = A v_j,
+//   w_j = tau_j y_j - (tau_j / 2) (tau_j y_j)^H v_j v_j,
+//   column j + 1 brought up to date (A -= v_j w_j^H + w_j v_j^H on it) and reflected by
+//   larfg's formulas (v_{j+1}, tau_{j+1}, beta = E[j+1]; the reflector's tail stored below
+//   the subdiagonal of column j + 1 and beta on it),
+//   one pass over A(j+2:, j+2:) applying the same update and forming y_{j+1} = A v_{j+1}
+//   (as a combination of columns, exact for the Hermitian A the updates keep).
+// D, E, tau and the stored reflectors are hetrd_lower's (the back-transformation reads them).
+constexpr int kTrdSmallThreads = 512;
+__global__ __launch_bounds__(kTrdSmallThreads) void trd_small_kernel(double2* __restrict__ A, int n,
+                                                                     double* __restrict__ Dg, double* __restrict__ Eg,
+                                                                     double2* __restrict__ tau) {
+    constexpr int NW = kTrdSmallThreads / 64, RU = kTrdSmallMax / 64;
+    const int b = blockIdx.x;
+    A += (int64_t)b * n * n;
+    Dg += (int64_t)b * n;
+    Eg += (int64_t)b * n;
+    tau += (int64_t)b * n;
+    __shared__ double2 vbuf[2][kTrdSmallMax], w0[kTrdSmallMax], y[kTrdSmallMax], col[kTrdSmallMax];
+    __shared__ double2 part[NW][kTrdSmallMax];
+    __shared__ double2 red2[NW];
+    __shared__ double red1[NW];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const double2 zero2 = make_double2(0.0, 0.0);
+    if (n <= 0) return;
+    // larfg on col[j + 1 ..) (every thread the same arithmetic from the reduced norm): beta into
+    // E and the subdiagonal, tau, the reflector into v (rows >= j + 1, v(j + 1) = 1) and below
+    // the subdiagonal of column j.  Returns tau_j; ends synchronised.
+    auto reflect = [&](int j, double2* v, double s2) -> double2 {
+        for (int off = 32; off > 0; off >>= 1) s2 += __shfl_xor(s2, off, 64);
+        if (lane == 0) red1[wv] = s2;
+        __syncthreads();
+        double xn2 = 0.0;
+        for (int q = 0; q < NW; ++q) xn2 += red1[q];
+        const double2 a = col[j + 1];
+        double beta, tr = 0.0, ti = 0.0;
+        double2 z = make_double2(1.0, 0.0);
+        if (xn2 == 0.0 && a.y == 0.0) {
+            beta = a.x;
+        } else {
+            beta = -copysign(sqrt(a.x * a.x + a.y * a.y + xn2), a.x);
+            tr = (beta - a.x) / beta;
+            ti = -a.y / beta;
+            const double zr = a.x - beta, zi = a.y, z2 = zr * zr + zi * zi;
+            z = make_double2(zr / z2, -zi / z2);
+        }
+        if (tid == 0) {
+            Eg[j] = beta;
+            tau[j] = make_double2(tr, ti);
+            A[(int64_t)j * n + j + 1] = make_double2(beta, 0.0);
+        }
+        for (int r = tid; r < n; r += kTrdSmallThreads) {
+            double2 vr = zero2;
+            if (r == j + 1) {
+                vr = make_double2(1.0, 0.0);
+            } else if (r > j + 1) {
+                vr = cmul(col[r], z);
+                A[(int64_t)j * n + r] = vr;
+            }
+            v[r] = vr;
+        }
+        __syncthreads();
+        return make_double2(tr, ti);
+    };
+    // y = A(s:, s:) v over rows and columns >= s, with the pending update (vp, w0) applied to
+    // A(s:, s:) first and written back when `upd`; each lane's rows of a column are loaded
+    // together before they are used
+    // y = A(s:, s:) v over rows and columns >= s as a combination of columns (A is Hermitian,
+    // so no reduction across lanes is needed): lane rows s + lane + 64 u, wave columns
+    // s + wv CG + g + NW CG t; the pending update (vp, w0) is applied to A(s:, s:) first and
+    // written back when `upd`.  The waves' partial vectors are summed in wave order.
+    auto pass = [&](int s, const double2* v, const double2* vp, bool upd) {
+        constexpr int CG = 4;  // columns per wave whose loads are issued before the first store
+        double2 acc[RU];
+#pragma unroll
+        for (int u = 0; u < RU; ++u) acc[u] = zero2;
+        for (int c0 = s + wv * CG; c0 < n; c0 += NW * CG) {
+            double2 a[CG][RU];
+#pragma unroll
+            for (int g = 0; g < CG; ++g)
+#pragma unroll
+                for (int u = 0; u < RU; ++u) {
+                    const int r = s + lane + 64 * u, c = c0 + g;
+                    a[g][u] = zero2;
+                    if (r < n && c < n) a[g][u] = A[(int64_t)c * n + r];
+                }
+#pragma unroll
+            for (int g = 0; g < CG; ++g) {
+                const int c = c0 + g;
+                if (c >= n) break;  // uniform over the wave
+                const double2 wc = upd ? w0[c] : zero2, vc = upd ? vp[c] : zero2, xc = v[c];
+#pragma unroll
+                for (int u = 0; u < RU; ++u) {
+                    const int r = s + lane + 64 * u;
+                    if (r < n) {
+                        double2 x = a[g][u];
+                        if (upd) {
+                            const double2 p = cmulc(vp[r], wc), q = cmulc(w0[r], vc);  // v(r) conj(w(c)) + w(r) conj(v(c))
+                            x = make_double2(x.x - p.x - q.x, x.y - p.y - q.y);
+                            A[(int64_t)c * n + r] = x;
+                        }
+                        const double2 e = cmul(x, xc);
+                        acc[u].x += e.x;
+                        acc[u].y += e.y;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < RU; ++u) {
+            const int r = s + lane + 64 * u;
+            if (r < n) part[wv][r] = acc[u];
+        }
+        __syncthreads();
+        for (int r = s + tid; r < n; r += kTrdSmallThreads) {
+            double2 t = part[0][r];
+            for (int q = 1; q < NW; ++q) {
+                t.x += part[q][r].x;
+                t.y += part[q][r].y;
+            }
+            y[r] = t;
+        }
+        __syncthreads();
+    };
+    // column 0
+    double s2 = 0.0;
+    for (int r = tid; r < n; r += kTrdSmallThreads) {
+        const double2 x = A[r];
+        col[r] = x;
+        if (r >= 2) s2 += x.x * x.x + x.y * x.y;
+    }
+    if (tid == 0) Dg[0] = A[0].x;
+    if (n == 1) return;
+    __syncthreads();
+    double2 tj = reflect(0, vbuf[0], s2);
+    pass(1, vbuf[0], vbuf[0], false);
+    for (int j = 0; j + 1 < n; ++j) {
+        const double2* v0 = vbuf[j & 1];
+        double2* v1 = vbuf[(j + 1) & 1];
+        // alpha = -(tau/2) (tau y)^H v over rows >= j + 1
+        double2 d = zero2;
+        for (int r = j + 1 + tid; r < n; r += kTrdSmallThreads) {
+            const double2 e = cmulc(v0[r], cmul(tj, y[r]));  // conj(tau y) v
+            d.x += e.x;
+            d.y += e.y;
+        }
+        d = wave_sum2(d);
+        if (lane == 0) red2[wv] = d;
+        __syncthreads();
+        double2 t = zero2;
+        for (int q = 0; q < NW; ++q) {
+            t.x += red2[q].x;
+            t.y += red2[q].y;
+        }
+        const double2 al = cmul(make_double2(-0.5 * tj.x, -0.5 * tj.y), t);
+        for (int r = j + 1 + tid; r < n; r += kTrdSmallThreads) {
+            const double2 x = cmul(tj, y[r]), av = cmul(al, v0[r]);
+            w0[r] = make_double2(x.x + av.x, x.y + av.y);
+        }
+        __syncthreads();
+        // column j + 1, up to date, and the norm of its tail
+        const int c1 = j + 1;
+        const double2 wc = w0[c1], vc = v0[c1];
+        s2 = 0.0;
+        for (int r = c1 + tid; r < n; r += kTrdSmallThreads) {
+            const double2 a = A[(int64_t)c1 * n + r];
+            const double2 p = cmulc(v0[r], wc), q = cmulc(w0[r], vc);
+            const double2 x = make_double2(a.x - p.x - q.x, a.y - p.y - q.y);
+            col[r] = x;
+            if (r == c1) Dg[c1] = x.x;
+            if (r >= c1 + 2) s2 += x.x * x.x + x.y * x.y;
+        }
+        if (c1 + 1 >= n) break;
+        __syncthreads();
+        tj = reflect(c1, v1, s2);
+        pass(c1 + 1, v1, v0, true);
+    }
+}
+
+void launch_trd_small(hipStream_t st, double2* A, int n, double* D, double* E, double2* tau, int batch) {
+    KV_REQUIRE(n >= 1 && n <= kTrdSmallMax && batch >= 1, ERR_INTERNAL, "trd_small: side above kTrdSmallMax");
+    hipLaunchKernelGGL(trd_small_kernel, dim3(batch), dim3(kTrdSmallThreads), 0, st, A, n, D, E, tau);
+    KV_HIP(hipGetLastError());
+}
+
 void launch_herm_fill(hipStream_t st, double2* A, int n, int batch) {
     if (n <= 0 || batch <= 0) return;
     const int T = (n + 31) / 32;

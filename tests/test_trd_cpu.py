@@ -139,3 +139,77 @@ def test_planted_mistake_is_caught(slip):
     assert max(np.abs(d - d0).max(), np.abs(e - e0).max()) < 1e-11
     _, d, e, _ = trd_blocked(np.tril(G), 4, slip=slip)
     assert max(np.abs(d - d0).max(), np.abs(e - e0).max()) > 1e-3
+
+
+def trd_fused(G, slip=None):
+    """trd_small_kernel's order: unblocked, each step's update A -= v w^H + w v^H applied to
+    the next column on the fly and to A(j+2:, j+2:) in the same pass that forms the next
+    y = A v.  Returns (A, d, e, tau) in zhetrd('L') storage."""
+    n = G.shape[0]
+    A = np.tril(G) + np.tril(G, -1).conj().T
+    A[np.diag_indices(n)] = A.diagonal().real
+    d = np.zeros(n)
+    e = np.zeros(max(n - 1, 0))
+    tau = np.zeros(max(n - 1, 0), complex)
+
+    def reflect(j, col):
+        beta, t, z = _larfg(col[j + 1], col[j + 2:])
+        e[j], tau[j] = beta, t
+        v = np.zeros(n, complex)
+        v[j + 1] = 1.0
+        v[j + 2:] = col[j + 2:] * z
+        A[j + 1, j] = beta
+        A[j + 2:, j] = v[j + 2:]
+        return v
+
+    d[0] = A[0, 0].real
+    if n == 1:
+        return A, d, e, tau
+    v = reflect(0, A[:, 0].copy())
+    y = np.zeros(n, complex)
+    y[1:] = A[1:, 1:] @ v[1:]
+    for j in range(n - 1):
+        r = slice(j + 1, n)
+        w = np.zeros(n, complex)
+        w[r] = tau[j] * y[r]
+        alpha = -0.5 * tau[j] * np.vdot(w[r], v[r])
+        if slip == "alpha_sign":
+            alpha = -alpha
+        w[r] += alpha * v[r]
+        c1 = j + 1
+        col = np.zeros(n, complex)
+        col[c1:] = A[c1:, c1] - v[c1:] * np.conj(w[c1]) - w[c1:] * np.conj(v[c1])
+        d[c1] = col[c1].real
+        if c1 + 1 >= n:
+            break
+        v1 = reflect(c1, col)
+        s = slice(c1 + 1, n)
+        A[s, s] -= np.outer(v[s], w[s].conj()) + np.outer(w[s], v[s].conj())
+        y = np.zeros(n, complex)
+        y[s] = A[s, s] @ v1[s]
+        v = v1
+    return A, d, e, tau
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 9, 64, 130])
+def test_fused_unblocked_reduction_equals_lapack(n):
+    """trd_small_kernel's restatement gives zhetrd('L')'s d, e, tau and reflector tails."""
+    rng = np.random.default_rng(n + 5)
+    G = _gram(rng, n, n + 2)
+    A, d, e, tau = trd_fused(np.tril(G))
+    c, d0, e0, t0, info = la.zhetrd(G, lower=1)
+    assert info == 0
+    scale = np.abs(G).max()
+    assert np.abs(d - d0).max() <= 1e-13 * n * scale
+    if n > 1:
+        assert np.abs(e - e0).max() <= 1e-13 * n * scale
+        assert np.abs(tau - t0[:n - 1]).max() <= 1e-12 * n
+        assert np.abs(np.tril(A, -2) - np.tril(c, -2)).max(initial=0.0) <= 1e-12 * n
+
+
+def test_fused_reduction_planted_sign_is_caught():
+    rng = np.random.default_rng(11)
+    G = _gram(rng, 20, 23)
+    c, d0, e0, t0, info = la.zhetrd(G, lower=1)
+    _, d, e, _ = trd_fused(np.tril(G), slip="alpha_sign")
+    assert max(np.abs(d - d0).max(), np.abs(e - e0).max()) > 1e-3
