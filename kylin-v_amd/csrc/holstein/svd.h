@@ -952,9 +952,10 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
     const int nj = (int)Y.size();
     KV_REQUIRE(b <= kCholMax, ERR_INTERNAL, "block route: block wider than kCholMax");
     // S_j = Y_j^H Y_j (one grouped GEMM), the left-looking panel Cholesky with T_j = R_j^{-1}
-    // (one launch per 32 columns, all jobs together), Q_j = Y_j T_j (one grouped GEMM)
+    // (two launches per 32 columns, all jobs together), Q_j = Y_j T_j (one grouped GEMM)
     const size_t bb = (size_t)b * b;
     DevBuf S(D.pool.get(), nj * bb * sizeof(double2)), T(D.pool.get(), nj * bb * sizeof(double2));
+    DevBuf Rd(D.pool.get(), (size_t)chol_left_rd_elems(b, nj) * sizeof(double2));
     DevBuf live(D.pool.get(), (size_t)nj * b * sizeof(int)), dmax(D.pool.get(), (size_t)nj * sizeof(double));
     DevBuf sz = upload_vec(D, std::vector<int>(nj, b));
     std::vector<int> hl((size_t)nj * b);
@@ -968,7 +969,7 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
             EventTimer ev;
             ev.start(D.stream);
             launch_chol_left(D.stream, S.as<double2>(), sz.as<int>(), b, b, (int64_t)bb, kBlockDrop, live.as<int>(),
-                             dmax.as<double>(), T.as<double2>(), nj);
+                             dmax.as<double>(), T.as<double2>(), Rd.as<double2>(), nj);
             ev.stop(D.stream);
             D.blk.orth_ms += ev.ms();
             D.blk.orth_calls += 1;

@@ -108,10 +108,14 @@ constexpr int kCholLeftNB = 32;
 // Batched left-looking blocked Cholesky S = R^H R (upper, row-major, ld, batch stride) with the
 // rank-tolerant pivot drop (tau * the largest diagonal entry, into dmax), forming T = R^{-1}
 // (same layout; dropped columns give zero rows and columns) and the live flags.  S's tiles
-// right of each diagonal block receive R; its diagonal tiles keep their input.  One launch per
-// 32-column panel.
+// right of each diagonal block receive R; its diagonal tiles keep their input.  Two launches
+// per 32-column panel; Rd (batch x ceil(ld / 32) x 32 x 32 complex) receives the inverted
+// diagonal blocks.
+inline int64_t chol_left_rd_elems(int ld, int batch) {
+    return (int64_t)batch * ((ld + kCholLeftNB - 1) / kCholLeftNB) * kCholLeftNB * kCholLeftNB;
+}
 void launch_chol_left(hipStream_t st, double2* S, const int* sizes, int nmax, int ld, int64_t stride, double tau,
-                      int* live, double* dmax, double2* T, int batch);
+                      int* live, double* dmax, double2* T, double2* Rd, int batch);
 // Columns c < ld of Y (n x ld, row-major) with live[c] == 0 refilled with seeded random values
 // whose squared column norm is about dmax[0].
 void launch_refill_cols(hipStream_t st, double2* Y, int n, int ld, const int* live, const double* dmax,

@@ -1748,23 +1748,182 @@ __global__ __launch_bounds__(256) void diag_max_kernel(const double2* __restrict
     if (threadIdx.x == 0) dmax[b] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
-// Panel step k of a left-looking blocked Cholesky S = R^H R (upper R, row-major, panels of
-// NB = kCholLeftNB columns) that also forms T = R^{-1}.  Every workgroup first forms the
-// panel's diagonal block C_kk = S_kk - R_{<k,k}^H R_{<k,k} and factors it in LDS, right-looking
-// with a rank-tolerant pivot rule: a pivot at or below tau * dmax marks its column dependent,
-// its row of R and of T are zero and it takes no part in the elimination (R is the factor of
-// S restricted to the live columns).  Every workgroup does the same arithmetic.  Then
-//   workgroup j < (panels from k on): column panel c = k + j; for j > 0
-//     R_kc = R_kk^{-H} (S_kc - R_{<k,k}^H R_{<k,c}) into S's tile (k, c) and a zero tile
-//     (c, k) of T; for j = 0 the inverse diagonal block into T's tile (k, k) and the live flags;
-//   workgroup j < k: T's tile (j, k) = -(T_{j,j..k-1} R_{j..k-1,k}) R_kk^{-1}.
-// Tiles of S written in step k (rows of panel k right of its diagonal block) are read by no
-// other workgroup of the step; the diagonal tile of S is left as it was (only T is used
-// afterwards).  grid (ceil(nmax / NB), batch), 256 threads; steps k = 0, 1, ... in order.
-__global__ __launch_bounds__(256) void chol_left_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
+// Left-looking blocked Cholesky S = R^H R with T = R^{-1}, panel k in two launches.
+// Shared by both: 32 x 32 tile sums over rows of R (or T) staged through LDS in 32-row
+// chunks, the four waves taking every fourth row and each lane a 4 x 4 block, the waves'
+// partial tiles added in wave order.
+struct CholTiles {
+    double2 At[kCholLeftNB][kCholLeftNB + 1], Bt[kCholLeftNB][kCholLeftNB + 1];
+    double2 red[3][kCholLeftNB][kCholLeftNB + 1];
+};
+
+// acc (this lane's 4 x 4 block of a 32 x 32 tile) += sum over rows i in [i_begin, i_end) of
+// op(A)(i, r) B(i, c), with A(i, r) = S[i ld + a_col + r] (conj, arows = false) or
+// T[(a_row + r) ld + i] (arows = true), B(i, c) = S[i ld + b_col + c] for c < bw; then the
+// waves' blocks are added into wave 0's.
+__device__ __forceinline__ void chol_tile_sum(CholTiles& L, double2 (&acc)[4][4], const double2* __restrict__ S,
+                                     const double2* __restrict__ T, int ld, int i_begin, int i_end, int a_col,
+                                     int a_row, bool arows, int aw, int b_col, int bw) {
+    constexpr int NB = kCholLeftNB;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r4 = (lane >> 3) * 4, c4 = (lane & 7) * 4;
+    const double2 zero2 = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) acc[x][y] = zero2;
+    for (int i0 = i_begin; i0 < i_end; i0 += NB) {
+        for (int e = tid; e < NB * NB; e += 256) {
+            const int ii = e / NB, cc = e % NB;
+            double2 va = zero2, vb = zero2;
+            if (arows)
+                va = T[(int64_t)(a_row + ii) * ld + i0 + cc];  // A^T staged: At[r][i]
+            else if (cc < aw)
+                va = S[(int64_t)(i0 + ii) * ld + a_col + cc];
+            if (cc < bw) vb = S[(int64_t)(i0 + ii) * ld + b_col + cc];
+            L.At[ii][cc] = va;
+            L.Bt[ii][cc] = vb;
+        }
+        __syncthreads();
+        for (int ii = wv; ii < NB; ii += 4) {
+            double2 av[4], bv[4];
+#pragma unroll
+            for (int x = 0; x < 4; ++x) {
+                av[x] = arows ? L.At[r4 + x][ii] : L.At[ii][r4 + x];
+                bv[x] = L.Bt[ii][c4 + x];
+            }
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y)
+                    acc[x][y] = arows ? cfma(acc[x][y], av[x], bv[y]) : cfma_conj(acc[x][y], av[x], bv[y]);
+        }
+        __syncthreads();
+    }
+    if (wv > 0)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) L.red[wv - 1][r4 + x][c4 + y] = acc[x][y];
+    __syncthreads();
+    if (wv == 0)
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+#pragma unroll
+            for (int x = 0; x < 4; ++x)
+#pragma unroll
+                for (int y = 0; y < 4; ++y) {
+                    const double2 v = L.red[q][r4 + x][c4 + y];
+                    acc[x][y].x += v.x;
+                    acc[x][y].y += v.y;
+                }
+    __syncthreads();
+}
+
+// Launch A of panel k, one workgroup per matrix: C_kk = S_kk - R_{<k,k}^H R_{<k,k}, its
+// factor with the rank-tolerant pivot rule (a pivot at or below tau * dmax drops its column:
+// R's row and T's row and column are zero; live flags), and its inverse Di, into T's tile
+// (k, k) and Rd[b][k].  The factor is right-looking in LDS with all threads; the inverse of
+// the triangular factor by recursive doubling: T_AC = -T_A R_AC T_C for pairs of diagonal
+// blocks of 1, 2, 4, 8 and 16.
+__global__ __launch_bounds__(256) void chol_diag_kernel(const double2* __restrict__ S, const int* __restrict__ sizes,
                                                         int ld, int64_t stride, double tau,
                                                         int* __restrict__ live, const double* __restrict__ dmax,
-                                                        double2* __restrict__ T, int k) {
+                                                        double2* __restrict__ T, double2* __restrict__ Rd, int k) {
+    constexpr int NB = kCholLeftNB;
+    const int b = blockIdx.x, n = sizes[b];
+    const int p0 = k * NB;
+    if (p0 >= n) return;  // uniform over the workgroup
+    const int w = min(NB, n - p0);
+    S += (int64_t)b * stride;
+    T += (int64_t)b * stride;
+    live += (int64_t)b * ld;
+    Rd += ((int64_t)b * ((ld + NB - 1) / NB) + k) * NB * NB;
+    const double thr = tau * dmax[b];
+    __shared__ CholTiles L;
+    __shared__ double2 Dg[NB][NB + 1], Ti[NB][NB + 1], Mt[NB][NB + 1];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int r4 = (lane >> 3) * 4, c4 = (lane & 7) * 4;
+    const double2 zero2 = make_double2(0.0, 0.0);
+    double2 ak[4][4];
+    chol_tile_sum(L, ak, S, T, ld, 0, p0, p0, 0, false, w, p0, w);
+    if (wv == 0)
+#pragma unroll
+        for (int x = 0; x < 4; ++x)
+#pragma unroll
+            for (int y = 0; y < 4; ++y) {
+                const int r = r4 + x, c = c4 + y;
+                double2 v = zero2;
+                if (r < w && c < w && c >= r) {
+                    const double2 sv = S[(int64_t)(p0 + r) * ld + p0 + c];
+                    v = make_double2(sv.x - ak[x][y].x, sv.y - ak[x][y].y);
+                }
+                Dg[r][c] = v;
+            }
+    __syncthreads();
+    // factor in LDS, right-looking, all threads: per pivot, the row scaled by 32 threads, then
+    // the trailing upper triangle updated (two elements per thread)
+    unsigned lv = 0u;
+    for (int kk = 0; kk < w; ++kk) {
+        const double piv = Dg[kk][kk].x;
+        const bool lk = piv > thr && piv > 0.0;  // NaN is not live
+        const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
+        lv |= lk ? 1u << kk : 0u;
+        __syncthreads();
+        if (tid >= kk && tid < w) {
+            const double2 v = Dg[kk][tid];
+            Dg[kk][tid] = tid == kk ? make_double2(lk ? sqrt(piv) : 0.0, 0.0) : make_double2(v.x * ri, v.y * ri);
+        }
+        __syncthreads();
+        if (lk)
+            for (int e = tid; e < NB * NB; e += 256) {
+                const int i = e / NB, c = e % NB;
+                if (i > kk && c >= i && c < w) {
+                    const double2 x = Dg[kk][i], y = Dg[kk][c];
+                    Dg[i][c] = make_double2(Dg[i][c].x - (x.x * y.x + x.y * y.y), Dg[i][c].y - (x.x * y.y - x.y * y.x));
+                }
+            }
+        __syncthreads();
+    }
+    // Ti = diag(1 / R_ii) over live indices
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int i = e / NB, c = e % NB;
+        Ti[i][c] = (i == c && i < w && ((lv >> i) & 1u)) ? make_double2(1.0 / Dg[i][i].x, 0.0) : zero2;
+    }
+    __syncthreads();
+    // recursive doubling: for pairs (A = [p, p+s), C = [p+s, p+2s)): M = R_AC T_C, T_AC = -T_A M
+    for (int s = 1; s < NB; s *= 2) {
+        const int per = s * s, tot = (NB / (2 * s)) * per;  // outputs per pair, all pairs
+        for (int e = tid; e < tot; e += 256) {
+            const int pr = e / per, rr = (e % per) / s, cc = e % s, p = pr * 2 * s;
+            double2 m = zero2;
+            for (int t = 0; t <= cc; ++t) m = cfma(m, Dg[p + rr][p + s + t], Ti[p + s + t][p + s + cc]);
+            Mt[p + rr][p + s + cc] = m;
+        }
+        __syncthreads();
+        for (int e = tid; e < tot; e += 256) {
+            const int pr = e / per, rr = (e % per) / s, cc = e % s, p = pr * 2 * s;
+            double2 m = zero2;
+            for (int t = rr; t < s; ++t) m = cfma(m, Ti[p + rr][p + t], Mt[p + t][p + s + cc]);
+            Ti[p + rr][p + s + cc] = make_double2(-m.x, -m.y);
+        }
+        __syncthreads();
+    }
+    for (int e = tid; e < NB * NB; e += 256) {
+        const int i = e / NB, c = e % NB;
+        Rd[e] = Ti[i][c];
+        if (i < w && c < w) T[(int64_t)(p0 + i) * ld + p0 + c] = Ti[i][c];
+    }
+    if (tid < w) live[p0 + tid] = (lv >> tid) & 1u;
+}
+
+// Launch B of panel k, grid (ceil(nmax / 32), batch): workgroup j with 0 < j < (panels from k
+// on) writes R's tile (k, k + j) = Di^H (S_kc - R_{<k,k}^H R_{<k,c}) into S and a zero tile
+// (k + j, k) into T; workgroup j < k writes T's tile (j, k) = -(T_{j, j..k-1} R_{j..k-1, k}) Di.
+// Tiles of S and T written here are read by no other workgroup of the launch.
+__global__ __launch_bounds__(256) void chol_panel_kernel(double2* __restrict__ S, const int* __restrict__ sizes,
+                                                         int ld, int64_t stride, double2* __restrict__ T,
+                                                         const double2* __restrict__ Rd, int k) {
     constexpr int NB = kCholLeftNB;
     const int b = blockIdx.y, n = sizes[b];
     const int p0 = k * NB;
@@ -1772,162 +1931,78 @@ __global__ __launch_bounds__(256) void chol_left_kernel(double2* __restrict__ S,
     const int w = min(NB, n - p0);
     const int j = blockIdx.x;
     const int npk = (n - p0 + NB - 1) / NB;
-    const bool chol_role = j < npk;
-    const bool inv_role = j < k;
-    if (!chol_role && !inv_role) return;  // uniform over the workgroup
+    const bool off = j > 0 && j < npk;
+    const bool inv = j < k;
+    if (!off && !inv) return;  // uniform over the workgroup
     S += (int64_t)b * stride;
     T += (int64_t)b * stride;
-    live += (int64_t)b * ld;
-    const double thr = tau * dmax[b];
-    __shared__ double2 At[NB][NB + 1], Bt[NB][NB + 1], Ct[NB][NB + 1];
-    __shared__ double2 Dg[NB][NB + 1], Di[NB][NB + 1];
-    __shared__ int lv_s[NB];
+    Rd += ((int64_t)b * ((ld + NB - 1) / NB) + k) * NB * NB;
+    __shared__ CholTiles L;
+    __shared__ double2 Di[NB][NB + 1], Ct[NB][NB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int r = tid >> 3, c4 = (tid & 7) * 4;  // this thread's 4 outputs of a 32 x 32 tile
+    const int r4 = (lane >> 3) * 4, c4 = (lane & 7) * 4;
     const double2 zero2 = make_double2(0.0, 0.0);
-    const int c0 = p0 + j * NB;                       // chol role: column panel
-    const int wc = chol_role ? min(NB, n - c0) : 0;
-    const bool off = chol_role && j > 0;
-    // C_kk and C_kc: the original tiles minus the rows of R above panel k
-    double2 ak[4], ac[4];
+    for (int e = tid; e < NB * NB; e += 256) Di[e / NB][e % NB] = Rd[e];
+    const int r = tid >> 3, cq = (tid & 7) * 4;
+    if (off) {
+        const int c0 = p0 + j * NB, wc = min(NB, n - c0);
+        double2 ac[4][4];
+        chol_tile_sum(L, ac, S, T, ld, 0, p0, p0, 0, false, w, c0, wc);
+        if (wv == 0)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) ak[q] = ac[q] = zero2;
-    for (int i0 = 0; i0 < p0; i0 += NB) {
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int ii = e / NB, cc = e % NB;
-            double2 va = zero2, vb = zero2;
-            if (cc < w) va = S[(int64_t)(i0 + ii) * ld + p0 + cc];
-            if (off && cc < wc) vb = S[(int64_t)(i0 + ii) * ld + c0 + cc];
-            At[ii][cc] = va;
-            Bt[ii][cc] = vb;
-        }
-        __syncthreads();
-#pragma unroll 4
-        for (int ii = 0; ii < NB; ++ii) {
-            const double2 a = At[ii][r];
+            for (int x = 0; x < 4; ++x)
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                ak[q] = cfma_conj(ak[q], a, At[ii][c4 + q]);
-                if (off) ac[q] = cfma_conj(ac[q], a, Bt[ii][c4 + q]);
-            }
-        }
-        __syncthreads();
-    }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        const int c = c4 + q;
-        Dg[r][c] = zero2;
-        Di[r][c] = zero2;
-        Ct[r][c] = zero2;
-        if (r < w && c < w && c >= r) {
-            const double2 sv = S[(int64_t)(p0 + r) * ld + p0 + c];
-            Dg[r][c] = make_double2(sv.x - ak[q].x, sv.y - ak[q].y);
-        }
-        if (off && r < w && c < wc) {
-            const double2 sv = S[(int64_t)(p0 + r) * ld + c0 + c];
-            Ct[r][c] = make_double2(sv.x - ac[q].x, sv.y - ac[q].y);
-        }
-    }
-    __syncthreads();
-    if (wv == 0) {  // factor and invert the diagonal block; one wave's LDS ops stay in order
-        for (int kk = 0; kk < w; ++kk) {
-            const double piv = Dg[kk][kk].x;
-            const bool lk = piv > thr && piv > 0.0;  // NaN is not live
-            const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
-            __builtin_amdgcn_wave_barrier();
-            if (lane > kk && lane < w) Dg[kk][lane] = make_double2(Dg[kk][lane].x * ri, Dg[kk][lane].y * ri);
-            if (lane == kk) {
-                Dg[kk][kk] = make_double2(lk ? sqrt(piv) : 0.0, 0.0);
-                lv_s[kk] = lk ? 1 : 0;
-            }
-            __builtin_amdgcn_wave_barrier();
-            if (lk)
-                for (int e = lane; e < NB * NB; e += 64) {
-                    const int i = e / NB, jj = e % NB;
-                    if (i > kk && jj >= i && jj < w) {
-                        const double2 a = Dg[kk][i], c = Dg[kk][jj];
-                        Dg[i][jj] = make_double2(Dg[i][jj].x - (a.x * c.x + a.y * c.y),
-                                                 Dg[i][jj].y - (a.x * c.y - a.y * c.x));
+                for (int y = 0; y < 4; ++y) {
+                    const int rr = r4 + x, c = c4 + y;
+                    double2 v = zero2;
+                    if (rr < w && c < wc) {
+                        const double2 sv = S[(int64_t)(p0 + rr) * ld + c0 + c];
+                        v = make_double2(sv.x - ac[x][y].x, sv.y - ac[x][y].y);
                     }
+                    Ct[rr][c] = v;
                 }
-            __builtin_amdgcn_wave_barrier();
-        }
-        if (lane < w && lv_s[lane]) {  // column `lane` of the inverse, by back substitution
-            const int c = lane;
-            Di[c][c] = make_double2(1.0 / Dg[c][c].x, 0.0);
-            for (int i = c - 1; i >= 0; --i) {
-                double2 acc = zero2;
-                if (lv_s[i]) {
-                    for (int kk = i + 1; kk <= c; ++kk) acc = cfma(acc, Dg[i][kk], Di[kk][c]);
-                    acc = make_double2(-acc.x / Dg[i][i].x, -acc.y / Dg[i][i].x);
-                }
-                Di[i][c] = acc;
-            }
-        }
-    }
-    __syncthreads();
-    if (chol_role && j == 0) {
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int i = e / NB, jj = e % NB;
-            if (i < w && jj < w) T[(int64_t)(p0 + i) * ld + p0 + jj] = Di[i][jj];
-        }
-        if (tid < w) live[p0 + tid] = lv_s[tid];
-    }
-    if (off) {  // R_kc = Di^H C_kc, and the zero tile of T below the diagonal
+        __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = c4 + q;
-            double2 y = zero2;
-            for (int m = 0; m <= r; ++m) y = cfma_conj(y, Di[m][r], Ct[m][c]);
-            if (r < w && c < wc) S[(int64_t)(p0 + r) * ld + c0 + c] = y;
+            const int c = cq + q;
+            double2 yv = zero2;
+            for (int m = 0; m <= r; ++m) yv = cfma_conj(yv, Di[m][r], Ct[m][c]);
+            if (r < w && c < wc) S[(int64_t)(p0 + r) * ld + c0 + c] = yv;
             if (c < w && r < wc) T[(int64_t)(c0 + r) * ld + p0 + c] = zero2;
         }
+        __syncthreads();  // Ct is reused
     }
-    if (inv_role) {  // T_jk = -(T_{j, j..k-1} R_{j..k-1, k}) Di
+    if (inv) {
         const int R0 = j * NB;
-        double2 x[4];
+        double2 xa[4][4];
+        chol_tile_sum(L, xa, S, T, ld, R0, p0, 0, R0, true, NB, p0, w);
+        if (wv == 0)
 #pragma unroll
-        for (int q = 0; q < 4; ++q) x[q] = zero2;
-        __syncthreads();  // At, Bt are reused
-        for (int i0 = R0; i0 < p0; i0 += NB) {
-            for (int e = tid; e < NB * NB; e += 256) {
-                const int ii = e / NB, cc = e % NB;
-                double2 vb = zero2;
-                if (cc < w) vb = S[(int64_t)(i0 + ii) * ld + p0 + cc];  // R rows i0.., panel k
-                At[ii][cc] = T[(int64_t)(R0 + ii) * ld + i0 + cc];    // T rows of block j
-                Bt[ii][cc] = vb;
-            }
-            __syncthreads();
-#pragma unroll 4
-            for (int ii = 0; ii < NB; ++ii) {
-                const double2 a = At[r][ii];
+            for (int x = 0; x < 4; ++x)
 #pragma unroll
-                for (int q = 0; q < 4; ++q) x[q] = cfma(x[q], a, Bt[ii][c4 + q]);
-            }
-            __syncthreads();
-        }
-#pragma unroll
-        for (int q = 0; q < 4; ++q) Ct[r][c4 + q] = x[q];
+                for (int y = 0; y < 4; ++y) Ct[r4 + x][c4 + y] = xa[x][y];
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
-            const int c = c4 + q;
-            double2 y = zero2;
-            for (int m = 0; m <= c; ++m) y = cfma(y, Ct[r][m], Di[m][c]);
-            if (c < w) T[(int64_t)(R0 + r) * ld + p0 + c] = make_double2(-y.x, -y.y);
+            const int c = cq + q;
+            double2 yv = zero2;
+            for (int m = 0; m <= c; ++m) yv = cfma(yv, Ct[r][m], Di[m][c]);
+            if (c < w) T[(int64_t)(R0 + r) * ld + p0 + c] = make_double2(-yv.x, -yv.y);
         }
     }
 }
 
 void launch_chol_left(hipStream_t st, double2* S, const int* sizes, int nmax, int ld, int64_t stride, double tau,
-                      int* live, double* dmax, double2* T, int batch) {
+                      int* live, double* dmax, double2* T, double2* Rd, int batch) {
     KV_REQUIRE(nmax <= ld && batch >= 1, ERR_INTERNAL, "chol_left: side above the leading dimension");
     hipLaunchKernelGGL(diag_max_kernel, dim3(batch), dim3(256), 0, st, S, sizes, ld, stride, dmax);
     KV_HIP(hipGetLastError());
     const int np = (nmax + kCholLeftNB - 1) / kCholLeftNB;
     for (int k = 0; k < np; ++k) {
-        hipLaunchKernelGGL(chol_left_kernel, dim3(np, batch), dim3(256), 0, st, S, sizes, ld, stride, tau, live,
-                           dmax, T, k);
+        hipLaunchKernelGGL(chol_diag_kernel, dim3(batch), dim3(256), 0, st, S, sizes, ld, stride, tau, live, dmax, T,
+                           Rd, k);
+        KV_HIP(hipGetLastError());
+        hipLaunchKernelGGL(chol_panel_kernel, dim3(np, batch), dim3(256), 0, st, S, sizes, ld, stride, T, Rd, k);
         KV_HIP(hipGetLastError());
     }
 }

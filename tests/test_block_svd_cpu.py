@@ -3,7 +3,7 @@
 The device route (randomised range finder with power steps and Rayleigh-Ritz) has no
 counterpart in the reference, which calls LAPACK's full SVD per sector.  This restates its
 steps in numpy — Cholesky QR with pivots at or below 1e-12 x the largest dropped
-(chol_upper: the unblocked right-looking elimination; chol_left: chol_left_kernel's
+(chol_upper: the unblocked right-looking elimination; chol_left: the device's
 left-looking 32-column panels, which also form the inverse factor T, checked against the
 unblocked factor and its inverse by back substitution), the power steps
 Q -> orth(A^H Q) -> orth(A P), the Ritz pairs, their residuals and the ceiling on the values
@@ -55,8 +55,9 @@ def tri_inv(R, live):
 
 
 def chol_left(S, nb=32, tau=DROP):
-    """chol_left_kernel's order: per panel k of nb columns C = S_{k,k:} - R_{<k,k}^H R_{<k,k:};
-    its diagonal block factored right-looking with the same pivot drops and inverted (Di);
+    """chol_diag_kernel / chol_panel_kernel's order: per panel k of nb columns
+    C = S_{k,k:} - R_{<k,k}^H R_{<k,k:}; its diagonal block factored right-looking with the same
+    pivot drops and inverted by recursive doubling (Di);
     R_{k,c} = Di^H C_{k,c} right of it; T_{k,k} = Di and T_{<k,k} = -(T_{<k,<k} R_{<k,k}) Di.
     Returns (R, live, T)."""
     S = np.array(S, dtype=complex)
@@ -80,14 +81,21 @@ def chol_left(S, nb=32, tau=DROP):
             if lk:
                 for i in range(k + 1, w):
                     Dg[i, i:] -= np.conj(Dg[k, i]) * Dg[k, i:]
-        Di = np.zeros((w, w), dtype=complex)
-        for c in range(w):
-            if not lv[c]:
-                continue
-            Di[c, c] = 1 / Dg[c, c].real
-            for i in range(c - 1, -1, -1):
-                if lv[i]:
-                    Di[i, c] = -(Dg[i, i + 1:c + 1] @ Di[i + 1:c + 1, c]) / Dg[i, i].real
+        # the inverse by recursive doubling over a 32-wide block (the kernel's padding beyond w
+        # is zero and stays decoupled): T_AC = -T_A R_AC T_C for pairs of blocks of 1, 2, 4, ...
+        Rb = np.zeros((nb, nb), dtype=complex)
+        Rb[:w, :w] = np.triu(Dg)
+        Tb = np.zeros((nb, nb), dtype=complex)
+        for i in range(w):
+            if lv[i]:
+                Tb[i, i] = 1 / Rb[i, i].real
+        h = 1
+        while h < nb:
+            for q in range(0, nb, 2 * h):
+                ia, ic = slice(q, q + h), slice(q + h, q + 2 * h)
+                Tb[ia, ic] = -Tb[ia, ia] @ (Rb[ia, ic] @ Tb[ic, ic])
+            h *= 2
+        Di = Tb[:w, :w]
         live[p:p + w] = lv
         R[p:p + w, p:p + w] = np.triu(Dg)
         R[p:p + w, p + w:] = Di.conj().T @ C[:, w:]
