@@ -204,7 +204,7 @@ def rooflines(st, precision):
     if st["trd_gpu_s"] > 0:
         ach = st["trd_bytes"] / st["trd_gpu_s"] / 1e9
         out["svd_tridiagonalisation"] = {
-            "kernel": "trd_column + trd_matvec + panel zgemm (bond SVD full route)", "bound": "hbm",
+            "kernel": "trd_small_kernel (Rayleigh-Ritz, sides <= 256) / hetrd_lower (full route)", "bound": "hbm",
             "achieved": ach, "peak": PEAK["hbm_gbs"], "unit": "GB/s", "frac": ach / PEAK["hbm_gbs"],
             "columns": int(st["trd_columns"])}
     return out
