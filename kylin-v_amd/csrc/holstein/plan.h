@@ -203,6 +203,7 @@ struct WFoldPlan {
     std::vector<WTask> tasks;
     std::vector<WTerm> terms;
     int maxP = 0;
+    int64_t maxPQ = 0;  // largest task's P x Q
     double flop = 0;
 };
 
@@ -295,6 +296,8 @@ inline WFoldPlan plan_wfold(const Layout& X, int p, const Layout& W, const doubl
         const int Sp = O.shape[p], WR = O.shape[p + 1];
         KV_REQUIRE(Pd < (1ll << 31) && Qd < (1ll << 31), ERR_ARG, "block too large");
         F.maxP = std::max<int>(F.maxP, (int)Pd);
+        KV_REQUIRE(Pd * Qd < (1ll << 31), ERR_ARG, "MPO fold: block too large");
+        F.maxPQ = std::max<int64_t>(F.maxPQ, Pd * Qd);
         for (int sp = 0; sp < Sp; ++sp)
             for (int wr = 0; wr < WR; ++wr) {
                 WTask t{};
@@ -337,6 +340,7 @@ inline WFoldPlan plan_wfold(const Layout& X, int p, const Layout& W, const doubl
         t.term_begin = (int)F.terms.size();
         t.term_count = 0;
         F.maxP = std::max(F.maxP, 1);
+        F.maxPQ = std::max<int64_t>(F.maxPQ, t.Q);
         F.tasks.push_back(t);
     }
     F.out = std::move(out);

@@ -241,7 +241,7 @@ struct DevWFold {
             KV_HIP(hipEventRecord(a, dev.stream));
         }
         launch_wfold<T2>(dev.stream, in, out, tasks.as<WTask>(), (int)p.tasks.size(),
-                         terms.as<WTerm>(), p.maxP);
+                         terms.as<WTerm>(), p.maxPQ);
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
             log->pending.push_back({a, b, p.flop, 1, 0.0});

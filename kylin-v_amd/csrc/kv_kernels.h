@@ -22,9 +22,10 @@ void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const
                         float2* C, const GemmTile* tiles, int ntiles, const GemmGroup* groups,
                         const GemmSeg* segs, double alpha_re, double alpha_im, int cfg);
 
+// MPO fold; maxPQ = the largest task's P x Q (sets the workgroups per task).
 template <class T2>
 void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,
-                  const WTerm* terms, int maxP);
+                  const WTerm* terms, int64_t maxPQ);
 
 template <class T2>
 void launch_perm(hipStream_t st, const T2* in, T2* out, const PermTask* tasks, int ntasks,

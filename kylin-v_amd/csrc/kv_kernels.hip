@@ -568,6 +568,10 @@ void launch_gemm_bf16x3(hipStream_t st, int opA, int opB, const float2* A, const
 // ------------------------------------------------------------------------------------
 // MPO fold
 // ------------------------------------------------------------------------------------
+// One workgroup column per task (an output (s', w') row of a block, P x Q elements), the
+// task's P x Q elements spread over all threads of gridDim.y workgroups in row-major order
+// (consecutive lanes on consecutive q, so every lane works even when Q < 256); the terms
+// are summed in order, four loads issued together.
 template <class T2>
 __global__ __launch_bounds__(256) void wfold_kernel(const T2* __restrict__ in,
                                                     T2* __restrict__ out,
@@ -575,37 +579,53 @@ __global__ __launch_bounds__(256) void wfold_kernel(const T2* __restrict__ in,
                                                     const WTerm* __restrict__ terms) {
     using T = typename ScalarOf<T2>::T;
     const WTask tk = tasks[blockIdx.x];
-    for (int p = blockIdx.y; p < tk.P; p += gridDim.y) {
-        for (int q = threadIdx.x; q < tk.Q; q += blockDim.x) {
-            T re = 0, im = 0;
-            for (int t = 0; t < tk.term_count; ++t) {
-                const WTerm w = terms[tk.term_begin + t];
-                const T2 v = in[w.in_base + (int64_t)p * w.in_pstride + q];
-                const T wr = (T)w.re, wi = (T)w.im;
-                re += wr * v.x - wi * v.y;
-                im += wr * v.y + wi * v.x;
+    const WTerm* __restrict__ tt = terms + tk.term_begin;
+    const int nt = tk.term_count, Q = tk.Q;
+    const int PQ = tk.P * Q;
+    for (int e = blockIdx.y * 256 + threadIdx.x; e < PQ; e += gridDim.y * 256) {
+        const int p = e / Q, q = e - p * Q;
+        T re = 0, im = 0;
+        int t = 0;
+        for (; t + 4 <= nt; t += 4) {
+            T2 v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) v[u] = in[tt[t + u].in_base + (int64_t)p * tt[t + u].in_pstride + q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const T wr = (T)tt[t + u].re, wi = (T)tt[t + u].im;
+                re += wr * v[u].x - wi * v[u].y;
+                im += wr * v[u].y + wi * v[u].x;
             }
-            T2 o;
-            o.x = re;
-            o.y = im;
-            out[tk.out_base + (int64_t)p * tk.out_pstride + q] = o;
         }
+        for (; t < nt; ++t) {
+            const WTerm w = tt[t];
+            const T2 v = in[w.in_base + (int64_t)p * w.in_pstride + q];
+            const T wr = (T)w.re, wi = (T)w.im;
+            re += wr * v.x - wi * v.y;
+            im += wr * v.y + wi * v.x;
+        }
+        T2 o;
+        o.x = re;
+        o.y = im;
+        out[tk.out_base + (int64_t)p * tk.out_pstride + q] = o;
     }
 }
 
 template <class T2>
 void launch_wfold(hipStream_t st, const T2* in, T2* out, const WTask* tasks, int ntasks,
-                  const WTerm* terms, int maxP) {
+                  const WTerm* terms, int64_t maxPQ) {
     if (ntasks <= 0) return;
-    const int gy = maxP < 1 ? 1 : (maxP < 64 ? maxP : 64);
-    hipLaunchKernelGGL((wfold_kernel<T2>), dim3(ntasks, gy), dim3(256), 0, st, in, out, tasks,
+    int64_t gy = (maxPQ + 255) / 256;
+    if (gy > 64) gy = 64;
+    if (gy < 1) gy = 1;
+    hipLaunchKernelGGL((wfold_kernel<T2>), dim3(ntasks, (unsigned)gy), dim3(256), 0, st, in, out, tasks,
                        terms);
     KV_HIP(hipGetLastError());
 }
 template void launch_wfold<double2>(hipStream_t, const double2*, double2*, const WTask*, int,
-                                    const WTerm*, int);
+                                    const WTerm*, int64_t);
 template void launch_wfold<float2>(hipStream_t, const float2*, float2*, const WTask*, int,
-                                   const WTerm*, int);
+                                   const WTerm*, int64_t);
 
 // ------------------------------------------------------------------------------------
 // block permutation
