@@ -72,16 +72,18 @@ inline KrylovExit local_krylov(const std::shared_ptr<Device>& dev, int nsite, co
             stats->heff_flop += H.flop();
             stats->matvecs += 1;
         }
+        // beta = <h|v_{k-1}>, h -= beta v_{k-1}, alpha = <h|v_k>, h -= alpha v_k, |h|^2: the
+        // reference's order, each subtraction fused with the reduction after it
+        double2* h = hv.as<double2>();
+        double2* sb = reinterpret_cast<double2*>(s);
         if (k > 0) {
             double2* vkm = Vb + (int64_t)(k - 1) * n;
-            launch_dotc(D.stream, hv.as<double2>(), vkm, n, reinterpret_cast<double2*>(s),
-                        scratch.as<double>());
-            launch_axpy_neg_dev(D.stream, hv.as<double2>(), vkm, reinterpret_cast<double2*>(s), n);
+            launch_dotc(D.stream, h, vkm, n, sb, scratch.as<double>());
+            launch_axpy_dotc(D.stream, h, vkm, sb, vk, n, sb + 1, scratch.as<double>());
+        } else {
+            launch_dotc(D.stream, h, vk, n, sb + 1, scratch.as<double>());
         }
-        launch_dotc(D.stream, hv.as<double2>(), vk, n, reinterpret_cast<double2*>(s + 2),
-                    scratch.as<double>());
-        launch_axpy_neg_dev(D.stream, hv.as<double2>(), vk, reinterpret_cast<double2*>(s + 2), n);
-        launch_nrm2sq(D.stream, hv.as<double2>(), n, s + 4, scratch.as<double>());
+        launch_axpy_nrm2sq(D.stream, h, vk, sb + 1, n, s + 4, scratch.as<double>());
         KV_HIP(hipMemcpyAsync(host, s, 5 * sizeof(double), hipMemcpyDeviceToHost, D.stream));
         D.sync();
         if (log) log->resolve(kstats ? *kstats : D.stats);

@@ -267,7 +267,6 @@ inline std::unique_ptr<Tensor> transpose(const std::shared_ptr<Device>& dev, con
     DevBuf tasks = upload_vec(*dev, P.tasks);
     launch_perm<double2>(dev->stream, X.d(), Y->d(), tasks.as<PermTask>(), (int)P.tasks.size(),
                          P.maxsize);
-    dev->sync();  // tasks buffer is released on return
     return Y;
 }
 
@@ -297,7 +296,6 @@ inline std::unique_ptr<Tensor> relayout(const std::shared_ptr<Device>& dev, cons
         DevBuf d = upload_vec(*dev, cps);
         launch_copy2d<double2>(dev->stream, X.d(), Y->d(), d.as<Copy2D>(), (int)cps.size(),
                                maxrows, nullptr);
-        dev->sync();
     }
     return Y;
 }

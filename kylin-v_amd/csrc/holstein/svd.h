@@ -32,6 +32,7 @@
 #include <rocsolver/rocsolver.h>
 
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -136,7 +137,6 @@ inline void gemm_dense(Device& D, char opA, char opB, const double2* A, int lda,
     DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
     launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, C, dt.as<GemmTile>(),
                          (int)ts.size(), dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
-    D.sync();
 }
 
 struct SectorSvd {
@@ -266,7 +266,6 @@ inline void qr_reduce(Device& D, Solver& sol, SectorSvd& X, bool colside) {
         ok(rocsolver_zungqr(sol.h, c, r, r, a, c, t), "rocsolver_zungqr");
     else
         ok(rocsolver_zunglq(sol.h, c, r, c, a, c, t), "rocsolver_zunglq");
-    D.sync();
     X.reduced = true;
     X.full = colside ? c : r;
     X.Qf = std::move(A);
@@ -443,8 +442,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                                   hipMemcpyDeviceToDevice, D.stream));
     } else {
         taub = DevBuf(D.pool.get(), (size_t)N * nb * sizeof(double2));
-        EventTimer trd_ev;
-        bool trd_timed = false;
+        hipEvent_t trd_t0 = nullptr;
         if (roc_trd) {
             auto* tp = reinterpret_cast<rocblas_double_complex*>(taub.get());
             const rocblas_status st = rocsolver_zhetrd_strided_batched(
@@ -452,7 +450,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                 E.as<double>(), N, tp, N, nb);
             KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocsolver_zhetrd failed");
         } else {
-            trd_ev.start(D.stream);
+            trd_t0 = D.spans.mark(D.stream);
             if (N <= kTrdSmallMax) {  // one workgroup per matrix, no per-column launches
                 launch_herm_fill(D.stream, Gb.as<double2>(), N, nb);
                 launch_trd_small(D.stream, Gb.as<double2>(), N, W.as<double>(), E.as<double>(), taub.as<double2>(),
@@ -460,8 +458,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             } else {
                 hetrd_lower(D, sol, Gb.as<double2>(), N, nb, W.as<double>(), E.as<double>(), taub.as<double2>());
             }
-            trd_ev.stop(D.stream);
-            trd_timed = true;
+            D.spans.close(trd_t0, D.stream, &D.trd.ms);
         }
         // the tridiagonal forms, kept for a redo (dstedc overwrites both arrays)
         DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
@@ -476,8 +473,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
         read_info();
-        if (trd_timed) {
-            D.trd.ms += trd_ev.ms();
+        if (trd_t0) {
             for (int64_t m = 1; m < N; ++m) D.trd.bytes += 16.0 * (double)(m * (m + 1) / 2) * nb;
             D.trd.columns += (long)N * nb;
         }
@@ -640,7 +636,6 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
                              p + (size_t)b * sv + r0, n, y, bw, &one, z + r0, n));
         }
     }
-    D.sync();
     X.Ev = std::move(Z);
     X.ev_lo = lo;
 }
@@ -700,7 +695,6 @@ inline void gram_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
         gemm_dense(D, 'N', 'N', B2.as<double2>(), n, m, (int)X.nc, X.Vh.as<double2>(), (int)X.nc,
                    nk, (int)X.nc, n);
     }
-    D.sync();
 }
 
 // Isometric route, second pass over the tail: the eigenvectors of G with values below
@@ -752,7 +746,6 @@ inline void iso_refine(Device& D, Solver& sol, SectorSvd& X, double tail, int mo
     gemm_dense(D, 'N', 'N', T.Ev.as<double2>(), nt, Bt, n, Bn.as<double2>(), n, nt, n, nt);
     KV_HIP(hipMemcpyAsync(Bt, Bn.get(), (size_t)nt * n * sizeof(double2), hipMemcpyDeviceToDevice,
                           D.stream));
-    D.sync();
     for (int j = 0; j < kmax - na; ++j) X.s[na + j] = T.s[j];  // the rest is null space
 }
 
@@ -796,7 +789,6 @@ inline void iso_vectors(Device& D, Solver& sol, SectorSvd& X, int nk) {
         gemm_dense(D, 'N', 'C', m, (int)X.nc, B.as<double2>(), n, X.U.as<double2>(), nk,
                    (int)X.nr, nk, (int)X.nc);
     }
-    D.sync();
 }
 
 // ---- block subspace route -------------------------------------------------------------
@@ -837,21 +829,44 @@ struct GemmJob {
 // offsets are measured from the lowest address among the jobs' operands, so their buffers may
 // be separate allocations.  When the outputs together have too few 64 x 64 tiles to fill the
 // GPU, K is cut into chunks whose partial products a second kernel sums in chunk order.
+// Tile config of the block-route products and the workgroup count the K split aims for:
+// KV_BLOCK_GEMM="cfg,wg" (cfg an index of kGemmConfigs built for the op pairs CN, NN, NC).
+struct BlockGemmTuning {
+    int cfg = 0, wg = 512;
+    static const BlockGemmTuning& get() {
+        static const BlockGemmTuning t = [] {
+            BlockGemmTuning o;
+            if (const char* e = std::getenv("KV_BLOCK_GEMM")) {
+                int c = 0, w = 0;
+                if (std::sscanf(e, "%d,%d", &c, &w) == 2 && c >= 0 && c < kNumGemmConfigs && w > 0) {
+                    o.cfg = c;
+                    o.wg = w;
+                }
+            }
+            return o;
+        }();
+        return t;
+    }
+};
+
 inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>& jobs) {
     if (jobs.empty()) return;
+    const BlockGemmTuning& tune = BlockGemmTuning::get();
+    const int cfg = gemm_config_supports(tune.cfg, op_code(opA), op_code(opB), true) ? tune.cfg : 0;
+    const int BM = kGemmConfigs[cfg].bm, BN = kGemmConfigs[cfg].bn;
     int tiles = 0, kmin = jobs[0].k;
     const double2 *a0 = jobs[0].A, *b0 = jobs[0].B;
     double2* cmin = jobs[0].C;
     for (const auto& j : jobs) {
         KV_REQUIRE(j.k > 0 && j.m > 0 && j.n > 0, ERR_INTERNAL, "block route: empty product");
-        tiles += ((j.m + GEMM_BM - 1) / GEMM_BM) * ((j.n + GEMM_BN - 1) / GEMM_BN);
+        tiles += ((j.m + BM - 1) / BM) * ((j.n + BN - 1) / BN);
         kmin = std::min(kmin, j.k);
         a0 = std::min(a0, j.A);
         b0 = std::min(b0, j.B);
         cmin = std::min(cmin, j.C);
     }
     int nch = 1;
-    while (tiles * nch < 512 && kmin / (2 * nch) >= 256) nch *= 2;
+    while (tiles * nch < tune.wg && kmin / (2 * nch) >= 256) nch *= 2;
     size_t part_elems = 0;
     if (nch > 1)
         for (const auto& j : jobs) part_elems += (size_t)nch * j.m * j.n;
@@ -887,28 +902,25 @@ inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>&
             const int gi = (int)gs.size();
             gs.push_back(g);
             ss.push_back(sg);
-            for (int a = 0; a < (j.m + GEMM_BM - 1) / GEMM_BM; ++a)
-                for (int b = 0; b < (j.n + GEMM_BN - 1) / GEMM_BN; ++b) ts.push_back(GemmTile{gi, a, b, 0});
+            for (int a = 0; a < (j.m + BM - 1) / BM; ++a)
+                for (int b = 0; b < (j.n + BN - 1) / BN; ++b) ts.push_back(GemmTile{gi, a, b, 0});
         }
         if (nch > 1) pos += (int64_t)nch * j.m * j.n;
         flop += 8.0 * j.m * j.n * (double)j.k;
         bytes += 16.0 * ((double)j.m * j.k + (double)j.k * j.n + (double)j.m * j.n);
     }
     DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
-    EventTimer ev;
-    ev.start(D.stream);
+    hipEvent_t t0 = D.spans.mark(D.stream);
     launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), a0, b0, c0, dt.as<GemmTile>(), (int)ts.size(),
-                         dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
+                         dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0, cfg);
     if (nch > 1)
         for (size_t q = 0; q < jobs.size(); ++q)
             launch_sum_parts(D.stream, part.as<double2>() + poff[q], nch, (int64_t)jobs[q].m * jobs[q].n,
                              jobs[q].C);
-    ev.stop(D.stream);
-    D.blk.gemm_ms += ev.ms();
+    D.spans.close(t0, D.stream, &D.blk.gemm_ms);
     D.blk.gemm_flop += flop;
     D.blk.gemm_bytes += bytes;
     D.blk.gemm_launches += 1;
-    D.sync();
 }
 
 // One sector of a bond in the block route.  A is M (isometry on the rows) or M^H (on the
@@ -966,12 +978,10 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
                 g.push_back(GemmJob{Y[q]->as<double2>(), b, Y[q]->as<double2>(), b, S.as<double2>() + q * bb, b, b,
                                     n[q]});
             gemm_jobs(D, 'C', 'N', g);
-            EventTimer ev;
-            ev.start(D.stream);
+            hipEvent_t t0 = D.spans.mark(D.stream);
             launch_chol_left(D.stream, S.as<double2>(), sz.as<int>(), b, b, (int64_t)bb, kBlockDrop, live.as<int>(),
                              dmax.as<double>(), T.as<double2>(), Rd.as<double2>(), nj);
-            ev.stop(D.stream);
-            D.blk.orth_ms += ev.ms();
+            D.spans.close(t0, D.stream, &D.blk.orth_ms);
             D.blk.orth_calls += 1;
             KV_HIP(hipMemcpyAsync(hl.data(), live.get(), hl.size() * sizeof(int), hipMemcpyDeviceToHost, D.stream));
             D.sync();
@@ -1077,14 +1087,12 @@ inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool i
         launch_zaxpby(D.stream, H[q].G.as<double2>(), H[q].G.as<double2>(), nullptr,
                       make_double2(1.0 / J[q]->f2, 0.0), make_double2(0.0, 0.0), (int64_t)b * b);
     {
-        EventTimer ev;
-        ev.start(D.stream);
+        hipEvent_t t0 = D.spans.mark(D.stream);
         // the split route (one-workgroup tridiagonalisation, dstedc, back-transformation):
         // rocSOLVER's zheevd spends most of its time reducing a 256-wide matrix column by column
         gram_solve(D, sol, hp, mode == EIG_HEEVD ? EIG_HEEVD : EIG_SPLIT, roc_trd);
         for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
-        ev.stop(D.stream);
-        D.blk.eig_ms += ev.ms();
+        D.spans.close(t0, D.stream, &D.blk.eig_ms);
         D.blk.eig_calls += 1;
     }
     std::vector<DevBuf> Vd;
@@ -1104,7 +1112,6 @@ inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool i
         DevBuf dc = upload_vec(D, cps);
         launch_copy2d<double2>(D.stream, H[q].Ev.as<double2>(), Vd.back().as<double2>(), dc.as<Copy2D>(), kc, 1,
                                nullptr);
-        D.sync();
     }
     std::vector<GemmJob> gz, gw;
     for (int q = 0; q < nj; ++q) {
@@ -1186,7 +1193,6 @@ inline void block_vectors(Device& D, SectorSvd& X, int nk) {
         launch_ctranspose(D.stream, X.Z.as<double2>(), X.Vh.as<double2>(), (int)X.nc, kc, kc, (int)X.nc);
         X.U = std::move(X.W);
     }
-    D.sync();
 }
 
 inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, const Tensor& A,
@@ -1256,7 +1262,6 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             DevBuf dc = upload_vec(D, cps);
             launch_copy2d<double2>(D.stream, A.d(), X[p].M.as<double2>(), dc.as<Copy2D>(),
                                    (int)cps.size(), maxrows, nullptr);
-            D.sync();
         }
         // sectors at least twice as wide as the block take the block subspace route
         if (block_ok && std::min(rs.len, cs.len) >= 2 * (int64_t)bw)
@@ -1510,7 +1515,6 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
                                (int)lc.size(), maxr_l, X[p].S.as<double>());
         launch_copy2d<double2>(D.stream, X[p].Vh.as<double2>(), res.right->d(), dr.as<Copy2D>(),
                                (int)rc.size(), maxr_r, X[p].S.as<double>());
-        D.sync();
         res.svals[p].assign(X[p].s.begin(), X[p].s.begin() + nk);
     }
     return res;

@@ -69,15 +69,12 @@ struct Tensor {
 
 // Host-to-device upload of a plan's descriptor array into a fresh device buffer.  The
 // copy is ordered on the context stream (the pooled buffer may still be read by earlier
-// work) and completed before returning, so the host vector may be freed afterwards.
+// work) and staged through the context's pinned ring, so the host vector may be freed at
+// once and the host does not wait.
 template <class T>
 inline DevBuf upload_vec(Device& dev, const std::vector<T>& v) {
     DevBuf b(dev.pool.get(), v.size() * sizeof(T));
-    if (!v.empty()) {
-        KV_HIP(hipMemcpyAsync(b.get(), v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice,
-                              dev.stream));
-        dev.sync();
-    }
+    dev.upload(b.get(), v.data(), v.size() * sizeof(T));
     return b;
 }
 

@@ -50,6 +50,13 @@ void launch_nrm2sq(hipStream_t st, const double2* a, int64_t n, double* out, dou
 // y -= c[0] * x   (c on the device)
 void launch_axpy_neg_dev(hipStream_t st, double2* y, const double2* x, const double2* c,
                          int64_t n);
+// y -= c[0] x (c on the device), then out = sum conj(y) z (complex) or sum |y|^2: the
+// subtraction and the reduction that follows it in one pass over y, with the arithmetic of
+// launch_axpy_neg_dev followed by launch_dotc / launch_nrm2sq.
+void launch_axpy_dotc(hipStream_t st, double2* y, const double2* x, const double2* c, const double2* z, int64_t n,
+                      double2* out, double* scratch);
+void launch_axpy_nrm2sq(hipStream_t st, double2* y, const double2* x, const double2* c, int64_t n, double* out,
+                        double* scratch);
 // dst = src / sqrt(nsq[0])  (nsq on the device)
 void launch_scale_invsqrt_dev(hipStream_t st, double2* dst, const double2* src,
                               const double* nsq, int64_t n);

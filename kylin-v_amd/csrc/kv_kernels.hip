@@ -877,6 +877,72 @@ void launch_axpy_neg_dev(hipStream_t st, double2* y, const double2* x, const dou
     KV_HIP(hipGetLastError());
 }
 
+// Lanczos passes fused with the subtraction that precedes them: y -= c[0] x, written back,
+// then per-workgroup partial sums of conj(y) z (DOT) or of |y|^2 for reduce2_final_kernel.
+// Element formulas, the element-to-block assignment and the reduction tree are those of
+// axpy_neg_dev_kernel followed by dotc_partial_kernel / nrm2sq_partial_kernel.
+template <bool DOT>
+__global__ __launch_bounds__(256) void axpy_reduce_partial_kernel(double2* __restrict__ y,
+                                                                  const double2* __restrict__ x,
+                                                                  const double2* __restrict__ c,
+                                                                  const double2* __restrict__ z, int64_t n,
+                                                                  double* __restrict__ part) {
+    __shared__ double sr[256], si[256];
+    const double2 cc = c[0];
+    double re = 0, im = 0;
+    for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+        const double2 v = x[e];
+        double2 o = y[e];
+        o.x -= cc.x * v.x - cc.y * v.y;
+        o.y -= cc.x * v.y + cc.y * v.x;
+        y[e] = o;
+        if constexpr (DOT) {
+            const double2 w = z[e];
+            re += o.x * w.x + o.y * w.y;  // conj(o) * w
+            im += o.x * w.y - o.y * w.x;
+        } else {
+            re += o.x * o.x + o.y * o.y;
+        }
+    }
+    sr[threadIdx.x] = re;
+    if constexpr (DOT) si[threadIdx.x] = im;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sr[threadIdx.x] += sr[threadIdx.x + s];
+            if constexpr (DOT) si[threadIdx.x] += si[threadIdx.x + s];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        if constexpr (DOT) {
+            part[2 * blockIdx.x] = sr[0];
+            part[2 * blockIdx.x + 1] = si[0];
+        } else {
+            part[blockIdx.x] = sr[0];
+        }
+    }
+}
+
+void launch_axpy_dotc(hipStream_t st, double2* y, const double2* x, const double2* c, const double2* z, int64_t n,
+                      double2* out, double* scratch) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(axpy_reduce_partial_kernel<true>, dim3(nb), dim3(256), 0, st, y, x, c, z, n, scratch);
+    KV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, scratch, nb, reinterpret_cast<double*>(out),
+                       2);
+    KV_HIP(hipGetLastError());
+}
+
+void launch_axpy_nrm2sq(hipStream_t st, double2* y, const double2* x, const double2* c, int64_t n, double* out,
+                        double* scratch) {
+    const int nb = red_blocks(n);
+    hipLaunchKernelGGL(axpy_reduce_partial_kernel<false>, dim3(nb), dim3(256), 0, st, y, x, c, nullptr, n, scratch);
+    KV_HIP(hipGetLastError());
+    hipLaunchKernelGGL(reduce2_final_kernel, dim3(1), dim3(256), 0, st, scratch, nb, out, 1);
+    KV_HIP(hipGetLastError());
+}
+
 __global__ __launch_bounds__(256) void scale_invsqrt_dev_kernel(double2* __restrict__ dst,
                                                                 const double2* __restrict__ src,
                                                                 const double* __restrict__ nsq,
@@ -1792,19 +1858,32 @@ __device__ __forceinline__ void chol_tile_sum(CholTiles& L, double2 (&acc)[4][4]
     for (int x = 0; x < 4; ++x)
 #pragma unroll
         for (int y = 0; y < 4; ++y) acc[x][y] = zero2;
-    for (int i0 = i_begin; i0 < i_end; i0 += NB) {
-        for (int e = tid; e < NB * NB; e += 256) {
-            const int ii = e / NB, cc = e % NB;
-            double2 va = zero2, vb = zero2;
+    // the next chunk's operands are loaded into registers while the current one is summed
+    constexpr int NQ = NB * NB / 256;
+    double2 pa[NQ], pb[NQ];
+    auto fetch = [&](int i0) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q, ii = e / NB, cc = e % NB;
+            pa[q] = zero2;
+            pb[q] = zero2;
             if (arows)
-                va = T[(int64_t)(a_row + ii) * ld + i0 + cc];  // A^T staged: At[r][i]
+                pa[q] = T[(int64_t)(a_row + ii) * ld + i0 + cc];  // A^T staged: At[r][i]
             else if (cc < aw)
-                va = S[(int64_t)(i0 + ii) * ld + a_col + cc];
-            if (cc < bw) vb = S[(int64_t)(i0 + ii) * ld + b_col + cc];
-            L.At[ii][cc] = va;
-            L.Bt[ii][cc] = vb;
+                pa[q] = S[(int64_t)(i0 + ii) * ld + a_col + cc];
+            if (cc < bw) pb[q] = S[(int64_t)(i0 + ii) * ld + b_col + cc];
+        }
+    };
+    if (i_begin < i_end) fetch(i_begin);
+    for (int i0 = i_begin; i0 < i_end; i0 += NB) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q;
+            L.At[e / NB][e % NB] = pa[q];
+            L.Bt[e / NB][e % NB] = pb[q];
         }
         __syncthreads();
+        if (i0 + NB < i_end) fetch(i0 + NB);
         for (int ii = wv; ii < NB; ii += 4) {
             double2 av[4], bv[4];
 #pragma unroll
@@ -1861,7 +1940,7 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* __restric
     Rd += ((int64_t)b * ((ld + NB - 1) / NB) + k) * NB * NB;
     const double thr = tau * dmax[b];
     __shared__ CholTiles L;
-    __shared__ double2 Dg[NB][NB + 1], Ti[NB][NB + 1], Mt[NB][NB + 1];
+    __shared__ double2 Dg[NB][NB + 1], Ti[NB][NB + 1], Rf[NB][NB + 1];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int r4 = (lane >> 3) * 4, c4 = (lane & 7) * 4;
     const double2 zero2 = make_double2(0.0, 0.0);
@@ -1879,52 +1958,66 @@ __global__ __launch_bounds__(256) void chol_diag_kernel(const double2* __restric
                     v = make_double2(sv.x - ak[x][y].x, sv.y - ak[x][y].y);
                 }
                 Dg[r][c] = v;
+                Rf[r][c] = zero2;
             }
     __syncthreads();
-    // factor in LDS, right-looking, all threads: per pivot, the row scaled by 32 threads, then
-    // the trailing upper triangle updated (two elements per thread)
+    // factor in LDS, right-looking, all threads, one phase per pivot: every thread reads the
+    // pivot; row kk of R (Dg's row over sqrt(piv)) goes to Rf and the trailing triangle takes
+    // the update with Dg's unscaled row over piv.  Dg's row kk is only read in the phase, and
+    // each thread loads all its operands before it stores.
+    constexpr int NQ = NB * NB / 256;
     unsigned lv = 0u;
     for (int kk = 0; kk < w; ++kk) {
         const double piv = Dg[kk][kk].x;
         const bool lk = piv > thr && piv > 0.0;  // NaN is not live
-        const double ri = lk ? 1.0 / sqrt(piv) : 0.0;
+        const double sp = lk ? sqrt(piv) : 0.0;
+        const double ri = lk ? 1.0 / sp : 0.0, rp = ri * ri;
         lv |= lk ? 1u << kk : 0u;
-        __syncthreads();
-        if (tid >= kk && tid < w) {
-            const double2 v = Dg[kk][tid];
-            Dg[kk][tid] = tid == kk ? make_double2(lk ? sqrt(piv) : 0.0, 0.0) : make_double2(v.x * ri, v.y * ri);
-        }
-        __syncthreads();
-        if (lk)
-            for (int e = tid; e < NB * NB; e += 256) {
-                const int i = e / NB, c = e % NB;
-                if (i > kk && c >= i && c < w) {
-                    const double2 x = Dg[kk][i], y = Dg[kk][c];
-                    Dg[i][c] = make_double2(Dg[i][c].x - (x.x * y.x + x.y * y.y), Dg[i][c].y - (x.x * y.y - x.y * y.x));
-                }
+        double2 xa[NQ], ya[NQ], za[NQ];
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q, i = e / NB, c = e % NB;
+            xa[q] = zero2;
+            za[q] = zero2;
+            ya[q] = Dg[kk][c];
+            if (i > kk && c >= i && c < w) {
+                xa[q] = Dg[kk][i];
+                za[q] = Dg[i][c];
             }
+        }
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            const int e = tid + 256 * q, i = e / NB, c = e % NB;
+            if (i == kk && c >= kk && c < w)
+                Rf[kk][c] = c == kk ? make_double2(sp, 0.0) : make_double2(ya[q].x * ri, ya[q].y * ri);
+            if (lk && i > kk && c >= i && c < w) {
+                const double2 x = xa[q], y = ya[q];
+                Dg[i][c] = make_double2(za[q].x - (x.x * y.x + x.y * y.y) * rp, za[q].y - (x.x * y.y - x.y * y.x) * rp);
+            }
+        }
         __syncthreads();
     }
     // Ti = diag(1 / R_ii) over live indices
     for (int e = tid; e < NB * NB; e += 256) {
         const int i = e / NB, c = e % NB;
-        Ti[i][c] = (i == c && i < w && ((lv >> i) & 1u)) ? make_double2(1.0 / Dg[i][i].x, 0.0) : zero2;
+        Ti[i][c] = (i == c && i < w && ((lv >> i) & 1u)) ? make_double2(1.0 / Rf[i][i].x, 0.0) : zero2;
     }
     __syncthreads();
     // recursive doubling: for pairs (A = [p, p+s), C = [p+s, p+2s)): M = R_AC T_C, T_AC = -T_A M
+    // (M in Dg, free after the factor)
     for (int s = 1; s < NB; s *= 2) {
         const int per = s * s, tot = (NB / (2 * s)) * per;  // outputs per pair, all pairs
         for (int e = tid; e < tot; e += 256) {
             const int pr = e / per, rr = (e % per) / s, cc = e % s, p = pr * 2 * s;
             double2 m = zero2;
-            for (int t = 0; t <= cc; ++t) m = cfma(m, Dg[p + rr][p + s + t], Ti[p + s + t][p + s + cc]);
-            Mt[p + rr][p + s + cc] = m;
+            for (int t = 0; t <= cc; ++t) m = cfma(m, Rf[p + rr][p + s + t], Ti[p + s + t][p + s + cc]);
+            Dg[p + rr][p + s + cc] = m;
         }
         __syncthreads();
         for (int e = tid; e < tot; e += 256) {
             const int pr = e / per, rr = (e % per) / s, cc = e % s, p = pr * 2 * s;
             double2 m = zero2;
-            for (int t = rr; t < s; ++t) m = cfma(m, Ti[p + rr][p + t], Mt[p + t][p + s + cc]);
+            for (int t = rr; t < s; ++t) m = cfma(m, Ti[p + rr][p + t], Dg[p + t][p + s + cc]);
             Ti[p + rr][p + s + cc] = make_double2(-m.x, -m.y);
         }
         __syncthreads();

@@ -6,6 +6,7 @@
 
 #include <cstdint>
 #include <cstdlib>
+#include <cstring>
 #include <map>
 #include <memory>
 #include <stdexcept>
@@ -177,10 +178,60 @@ struct BlockStats {
     long orth_calls = 0, eig_calls = 0, sectors = 0, fallbacks = 0;
 };
 
+// HIP-event spans on the context stream whose durations are added to their counters when
+// the stream is next synchronised, so timing a launch does not make the host wait for it.
+// Counters must outlive the pending spans (fields of the Device).
+class SpanLog {
+  public:
+    SpanLog() = default;
+    SpanLog(const SpanLog&) = delete;
+    SpanLog& operator=(const SpanLog&) = delete;
+    ~SpanLog() {
+        for (auto& p : pending_) {
+            (void)hipEventDestroy(p.a);
+            (void)hipEventDestroy(p.b);
+        }
+        for (hipEvent_t e : spare_) (void)hipEventDestroy(e);
+    }
+    hipEvent_t mark(hipStream_t s) {
+        hipEvent_t e;
+        if (!spare_.empty()) {
+            e = spare_.back();
+            spare_.pop_back();
+        } else {
+            KV_HIP(hipEventCreate(&e));
+        }
+        KV_HIP(hipEventRecord(e, s));
+        return e;
+    }
+    // closes the span opened by mark(); its milliseconds go to *ms_counter
+    void close(hipEvent_t a, hipStream_t s, double* ms_counter) { pending_.push_back({a, mark(s), ms_counter}); }
+    // call only after the stream has been synchronised
+    void resolve() {
+        for (auto& p : pending_) {
+            float ms = 0;
+            KV_HIP(hipEventElapsedTime(&ms, p.a, p.b));
+            *p.dst += ms;
+            spare_.push_back(p.a);
+            spare_.push_back(p.b);
+        }
+        pending_.clear();
+    }
+
+  private:
+    struct Span {
+        hipEvent_t a, b;
+        double* dst;
+    };
+    std::vector<Span> pending_;
+    std::vector<hipEvent_t> spare_;
+};
+
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
     std::unique_ptr<Pool> pool;
+    SpanLog spans;
     bool timing = false;  // record HIP events around contraction kernels
     KernelStats stats;
     TrdStats trd;
@@ -193,6 +244,7 @@ struct Device {
         KV_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         pool.reset(new Pool(dev));
         if (const char* a = std::getenv("KV_AUTOTUNE")) autotune = a[0] != '0';
+        KV_HIP(hipHostMalloc(&ring_, kRingBytes, hipHostMallocDefault));
     }
     ~Device() {
         if (stream) {
@@ -200,8 +252,37 @@ struct Device {
             pool.reset();
             (void)hipStreamDestroy(stream);
         }
+        if (ring_) (void)hipHostFree(ring_);
     }
-    void sync() { KV_HIP(hipStreamSynchronize(stream)); }
+    Device(const Device&) = delete;
+    Device& operator=(const Device&) = delete;
+    // Waits for the stream; then the upload ring is free again and pending spans resolve.
+    void sync() {
+        KV_HIP(hipStreamSynchronize(stream));
+        ring_off_ = 0;
+        spans.resolve();
+    }
+    // Host-to-device copy of a small host array (work descriptors) that returns without
+    // waiting: the bytes go through a pinned staging ring, whose slots are reused only after
+    // the next sync().  Copies larger than a quarter of the ring wait for themselves.
+    void upload(void* dst, const void* src, size_t bytes) {
+        if (!bytes) return;
+        if (bytes > kRingBytes / 4) {
+            KV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+            sync();
+            return;
+        }
+        if (ring_off_ + bytes > kRingBytes) sync();
+        char* slot = static_cast<char*>(ring_) + ring_off_;
+        std::memcpy(slot, src, bytes);
+        KV_HIP(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, stream));
+        ring_off_ = (ring_off_ + bytes + 255) / 256 * 256;
+    }
+
+  private:
+    static constexpr size_t kRingBytes = (size_t)8 << 20;
+    void* ring_ = nullptr;
+    size_t ring_off_ = 0;
 };
 
 // Times a region of stream work with a pair of HIP events recorded on the stream the

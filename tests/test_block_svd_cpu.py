@@ -70,21 +70,26 @@ def chol_left(S, nb=32, tau=DROP):
         w = min(nb, n - p)
         C = S[p:p + w, p:] - R[:p, p:p + w].conj().T @ R[:p, p:]
         Dg = np.triu(C[:, :w])
+        Rf = np.zeros((w, w), dtype=complex)
         lv = np.zeros(w, dtype=bool)
         for k in range(w):
+            # one phase per pivot: R's row is Dg's row over sqrt(piv); the trailing triangle
+            # takes the update with Dg's unscaled row over piv
             piv = Dg[k, k].real
             lk = piv > thr and piv > 0
             lv[k] = lk
-            ri = 1 / np.sqrt(piv) if lk else 0.0
-            Dg[k, k + 1:] *= ri
-            Dg[k, k] = np.sqrt(piv) if lk else 0.0
+            sp = np.sqrt(piv) if lk else 0.0
+            ri = 1 / sp if lk else 0.0
+            rp = ri * ri
+            Rf[k, k] = sp
+            Rf[k, k + 1:] = Dg[k, k + 1:] * ri
             if lk:
                 for i in range(k + 1, w):
-                    Dg[i, i:] -= np.conj(Dg[k, i]) * Dg[k, i:]
+                    Dg[i, i:] -= (np.conj(Dg[k, i]) * Dg[k, i:]) * rp
         # the inverse by recursive doubling over a 32-wide block (the kernel's padding beyond w
         # is zero and stays decoupled): T_AC = -T_A R_AC T_C for pairs of blocks of 1, 2, 4, ...
         Rb = np.zeros((nb, nb), dtype=complex)
-        Rb[:w, :w] = np.triu(Dg)
+        Rb[:w, :w] = Rf
         Tb = np.zeros((nb, nb), dtype=complex)
         for i in range(w):
             if lv[i]:
@@ -97,7 +102,7 @@ def chol_left(S, nb=32, tau=DROP):
             h *= 2
         Di = Tb[:w, :w]
         live[p:p + w] = lv
-        R[p:p + w, p:p + w] = np.triu(Dg)
+        R[p:p + w, p:p + w] = Rf
         R[p:p + w, p + w:] = Di.conj().T @ C[:, w:]
         T[p:p + w, p:p + w] = Di
         T[:p, p:p + w] = -(T[:p, :p] @ R[:p, p:p + w]) @ Di
