@@ -6,6 +6,7 @@
 // executors in product.h upload its arrays to the device once.
 #pragma once
 
+#include <array>
 #include <cmath>
 #include <map>
 #include <utility>
@@ -205,6 +206,9 @@ struct WFoldPlan {
     int maxP = 0;
     int64_t maxPQ = 0;  // largest task's P x Q
     double flop = 0;
+    // host-side provenance for compose_wfold: per task (output block, s', wR), -1s for a
+    // zero task; per term (input block, wL, s)
+    std::vector<std::array<int, 3>> task_src, term_src;
 };
 
 // MPO fold (the env_mps_mpo family, holstein/product.h:203-477):
@@ -322,11 +326,13 @@ inline WFoldPlan plan_wfold(const Layout& X, int p, const Layout& W, const doubl
                             w.re = re;
                             w.im = im;
                             F.terms.push_back(w);
+                            F.term_src.push_back({pr.first, wl, s});
                             F.flop += 8.0 * (double)Pd * (double)Qd;
                         }
                 }
                 t.term_count = (int)F.terms.size() - t.term_begin;
                 F.tasks.push_back(t);
+                F.task_src.push_back({ob, sp, wr});
             }
     }
     for (size_t ob = 0; ob < out.blocks.size(); ++ob) {
@@ -342,9 +348,77 @@ inline WFoldPlan plan_wfold(const Layout& X, int p, const Layout& W, const doubl
         F.maxP = std::max(F.maxP, 1);
         F.maxPQ = std::max<int64_t>(F.maxPQ, t.Q);
         F.tasks.push_back(t);
+        F.task_src.push_back({-1, -1, -1});
     }
     F.out = std::move(out);
     return F;
+}
+
+// The two folds of the two-site H_eff as one pass: F1 = plan_wfold(X, 1, W1) takes
+// X[pre, wL, s1, s2, post] to Y[pre, s1', w1, s2, post] and F2 = plan_wfold(Y, 2, W2) takes
+// Y to Z[pre, s1', s2', wR, post].  The composite reads X directly:
+//   Z[pre, s1', s2', wR, post] = sum_{w1, s2} W2[s2', w1, s2, wR]
+//                                  sum_{wL, s1} W1[s1', wL, s1, w1] X[pre, wL, s1, s2, post],
+// one task per (F2 task, s1') over P = pre, Q = post, with a term per pair of nonzero W
+// entries (weight W2 W1, X's slice of F1's term shifted to s2).  Z's layout is F2's.
+inline WFoldPlan compose_wfold(const WFoldPlan& F1, const WFoldPlan& F2, const Layout& Y) {
+    KV_REQUIRE(F1.term_src.size() == F1.terms.size() && F2.term_src.size() == F2.terms.size() &&
+                   F1.task_src.size() == F1.tasks.size() && F2.task_src.size() == F2.tasks.size(),
+               ERR_INTERNAL, "compose_wfold: plans without provenance");
+    WFoldPlan C;
+    C.out = F2.out;
+    // F1's task writing Y block yb at (s1', w1)
+    std::map<std::array<int, 3>, int> f1_task;
+    for (size_t t = 0; t < F1.tasks.size(); ++t)
+        if (F1.task_src[t][0] >= 0) f1_task[F1.task_src[t]] = (int)t;
+    for (size_t t2 = 0; t2 < F2.tasks.size(); ++t2) {
+        const WTask& T2 = F2.tasks[t2];
+        if (F2.task_src[t2][0] < 0) {  // an output block no term reaches: zeros
+            C.tasks.push_back(T2);
+            C.tasks.back().term_begin = (int)C.terms.size();
+            C.task_src.push_back({-1, -1, -1});
+            C.maxP = std::max(C.maxP, T2.P);
+            C.maxPQ = std::max<int64_t>(C.maxPQ, (int64_t)T2.P * T2.Q);
+            continue;
+        }
+        const BlockDesc& Ob = C.out.blocks[F2.task_src[t2][0]];
+        const int S1 = Ob.shape[1];  // s1' (F2's pre is (pre, s1'))
+        KV_REQUIRE(S1 > 0 && T2.P % S1 == 0, ERR_INTERNAL, "compose_wfold: pre does not end in s1'");
+        for (int s1 = 0; s1 < S1; ++s1) {
+            WTask t{};
+            t.out_base = T2.out_base + (int64_t)s1 * T2.out_pstride;
+            t.out_pstride = (int64_t)S1 * T2.out_pstride;
+            t.P = T2.P / S1;
+            t.Q = T2.Q;
+            t.term_begin = (int)C.terms.size();
+            for (int k2 = T2.term_begin; k2 < T2.term_begin + T2.term_count; ++k2) {
+                const WTerm& w2 = F2.terms[k2];
+                const auto& src = F2.term_src[k2];  // (Y block, w1, s2)
+                auto it = f1_task.find({src[0], s1, src[1]});
+                if (it == f1_task.end()) continue;  // that slice of Y is zero
+                const WTask& T1 = F1.tasks[it->second];
+                KV_REQUIRE(T1.P == t.P, ERR_INTERNAL, "compose_wfold: pre extents differ");
+                for (int k1 = T1.term_begin; k1 < T1.term_begin + T1.term_count; ++k1) {
+                    const WTerm& w1 = F1.terms[k1];
+                    WTerm w{};
+                    w.in_base = w1.in_base + (int64_t)src[2] * t.Q;
+                    w.in_pstride = w1.in_pstride;
+                    w.re = w2.re * w1.re - w2.im * w1.im;
+                    w.im = w2.re * w1.im + w2.im * w1.re;
+                    C.terms.push_back(w);
+                    C.term_src.push_back({-1, -1, -1});
+                    C.flop += 8.0 * (double)t.P * (double)t.Q;
+                }
+            }
+            t.term_count = (int)C.terms.size() - t.term_begin;
+            C.tasks.push_back(t);
+            C.task_src.push_back({-1, -1, -1});
+            C.maxP = std::max(C.maxP, t.P);
+            C.maxPQ = std::max<int64_t>(C.maxPQ, (int64_t)t.P * t.Q);
+        }
+    }
+    (void)Y;
+    return C;
 }
 
 struct PermPlan {
@@ -424,6 +498,7 @@ struct HeffStructure {
     Layout xl;
     ProductPlan s1, s4;
     WFoldPlan s2, s3;
+    WFoldPlan s23;  // two sites: both folds as one pass (compose_wfold)
     double flop = 0;
 };
 
@@ -455,6 +530,7 @@ inline HeffStructure plan_heff(int nsite, const Layout& L, const Layout* W1, con
     const Layout* t = nsite == 2 ? &H.s3.out : (nsite == 1 ? &H.s2.out : &H.s1.out);
     H.s4 = plan_product(*t, R, 2, 'N', 'T', &cur);
     H.xl = cur;
+    if (nsite == 2) H.s23 = compose_wfold(H.s2, H.s3, H.s2.out);
     H.flop = H.s1.flop + H.s2.flop + H.s3.flop + H.s4.flop;
     return H;
 }

@@ -323,7 +323,8 @@ struct Heff {
     HeffStructure S;
     Prec prec = PREC_F64;
     DevProduct s1, s4;
-    DevWFold s2, s3;
+    // the MPO folds; two sites in FP32 / split-BF16 also both folds as one pass (s23)
+    DevWFold s2, s3, s23;
     const Tensor* L = nullptr;
     const Tensor* R = nullptr;
     DevBuf L32, R32, x32, y32, t1, t2, t3;
@@ -341,6 +342,7 @@ struct Heff {
         s1.build(*dev, S.s1);
         if (nsite >= 1) s2.build(*dev, S.s2);
         if (nsite == 2) s3.build(*dev, S.s3);
+        if (nsite == 2 && p != PREC_F64) s23.build(*dev, S.s23);
         s4.build(*dev, S.s4);
         // intermediates sized for FP64 so either precision can run through them
         const size_t es = sizeof(double2);
@@ -364,13 +366,22 @@ struct Heff {
                bool split = false) {
         s1.run<T2>(*dev, Ld, x, t1.as<T2>(), 1.0, 0.0, log, split);
         const T2* last = t1.as<T2>();
-        if (S.nsite >= 1) {
-            s2.run<T2>(*dev, last, t2.as<T2>(), log);
-            last = t2.as<T2>();
-        }
-        if (S.nsite == 2) {
-            s3.run<T2>(*dev, last, t3.as<T2>(), log);
+        // FP64 keeps the two folds one after the other (the reference's order of sums);
+        // the FP32 operands take the one-pass composite, which saves a pass over the
+        // intermediate at the cost of a different rounding order
+        const bool one_pass = S.nsite == 2 && !std::is_same<T2, double2>::value;
+        if (one_pass) {
+            s23.run<T2>(*dev, last, t3.as<T2>(), log);
             last = t3.as<T2>();
+        } else {
+            if (S.nsite >= 1) {
+                s2.run<T2>(*dev, last, t2.as<T2>(), log);
+                last = t2.as<T2>();
+            }
+            if (S.nsite == 2) {
+                s3.run<T2>(*dev, last, t3.as<T2>(), log);
+                last = t3.as<T2>();
+            }
         }
         s4.run<T2>(*dev, last, Rd, y, alpha, 0.0, log, split);
     }

@@ -472,7 +472,25 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
                 E.as<double>() + (size_t)b * N, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
         }
-        read_info();
+        // info and the sum of squares of every sector's eigenvectors (NaN check), one wait
+        std::vector<double> zsq(nb, 0.0);
+        {
+            DevBuf zs(D.pool.get(), (size_t)nb * sizeof(double)), sc(D.pool.get(), 2048 * sizeof(double));
+            std::vector<double> tails(nb, 0.0);
+            for (int b = 0; b < nb; ++b) {
+                const int64_t cnt = (int64_t)xs[b]->k * xs[b]->k;
+                const double* z = xs[b]->Zt.as<double>();
+                launch_nrm2sq(D.stream, reinterpret_cast<const double2*>(z), cnt / 2, zs.as<double>() + b,
+                              sc.as<double>());
+                if (cnt % 2)
+                    KV_HIP(hipMemcpyAsync(&tails[b], z + cnt - 1, sizeof(double), hipMemcpyDeviceToHost, D.stream));
+            }
+            KV_HIP(hipMemcpyAsync(zsq.data(), zs.get(), nb * sizeof(double), hipMemcpyDeviceToHost, D.stream));
+            KV_HIP(hipMemcpyAsync(hinfo.data(), info.get(), nb * sizeof(rocblas_int), hipMemcpyDeviceToHost,
+                                  D.stream));
+            D.sync();
+            for (int b = 0; b < nb; ++b) zsq[b] += tails[b] * tails[b];
+        }
         if (trd_t0) {
             for (int64_t m = 1; m < N; ++m) D.trd.bytes += 16.0 * (double)(m * (m + 1) / 2) * nb;
             D.trd.columns += (long)N * nb;
@@ -480,7 +498,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
         for (int b = 0; b < nb; ++b) {
             const int n = xs[b]->k;
             const int64_t nn = (int64_t)n * n;
-            const bool ok = hinfo[b] == 0 && std::isfinite(dev_sumsq(D, xs[b]->Zt.as<double>(), nn));
+            const bool ok = hinfo[b] == 0 && std::isfinite(zsq[b]);
             if (ok) continue;
             KV_HIP(hipMemcpyAsync(W.as<double>() + (size_t)b * N, W0.as<double>() + (size_t)b * N, n * sizeof(double),
                                   hipMemcpyDeviceToDevice, D.stream));
@@ -829,18 +847,24 @@ struct GemmJob {
 // offsets are measured from the lowest address among the jobs' operands, so their buffers may
 // be separate allocations.  When the outputs together have too few 64 x 64 tiles to fill the
 // GPU, K is cut into chunks whose partial products a second kernel sums in chunk order.
-// Tile config of the block-route products and the workgroup count the K split aims for:
-// KV_BLOCK_GEMM="cfg,wg" (cfg an index of kGemmConfigs built for the op pairs CN, NN, NC).
+// Tile config of the block-route products, the workgroup count the K split aims for, the
+// shortest K chunk it cuts and the tile order: KV_BLOCK_GEMM="cfg,wg[,kchunk[,xcd]]" (cfg an
+// index of kGemmConfigs built for the op pairs CN, NN, NC; xcd 1 orders each product's tiles
+// so that the tiles of one row of tiles, which read the same panel of the large operand, go
+// to one XCD: workgroups are dealt to the 8 XCDs round-robin).
 struct BlockGemmTuning {
-    int cfg = 0, wg = 512;
+    int cfg = 0, wg = 768, kchunk = 64, xcd = 0;
     static const BlockGemmTuning& get() {
         static const BlockGemmTuning t = [] {
             BlockGemmTuning o;
             if (const char* e = std::getenv("KV_BLOCK_GEMM")) {
-                int c = 0, w = 0;
-                if (std::sscanf(e, "%d,%d", &c, &w) == 2 && c >= 0 && c < kNumGemmConfigs && w > 0) {
+                int c = 0, w = 0, k = 64, x = 0;
+                const int got = std::sscanf(e, "%d,%d,%d,%d", &c, &w, &k, &x);
+                if (got >= 2 && c >= 0 && c < kNumGemmConfigs && w > 0 && k >= 32) {
                     o.cfg = c;
                     o.wg = w;
+                    o.kchunk = k;
+                    o.xcd = x;
                 }
             }
             return o;
@@ -866,7 +890,7 @@ inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>&
         cmin = std::min(cmin, j.C);
     }
     int nch = 1;
-    while (tiles * nch < tune.wg && kmin / (2 * nch) >= 256) nch *= 2;
+    while (tiles * nch < tune.wg && kmin / (2 * nch) >= tune.kchunk && 2 * nch <= tune.kchunk) nch *= 2;
     size_t part_elems = 0;
     if (nch > 1)
         for (const auto& j : jobs) part_elems += (size_t)nch * j.m * j.n;
@@ -880,8 +904,10 @@ inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>&
     double flop = 0, bytes = 0;
     for (size_t q = 0; q < jobs.size(); ++q) {
         const GemmJob& j = jobs[q];
-        // chunks of at least 512 deep (kmin >= 512 nch), so none is empty after rounding
-        const int kc = nch == 1 ? j.k : ((j.k + nch - 1) / nch + 7) / 8 * 8;
+        // chunk depth ceil(k / nch), rounded up to a multiple of 8 where that leaves the last
+        // chunk nonempty (k / nch >= kchunk >= nch keeps ceil's chunks nonempty)
+        const int kq = (j.k + nch - 1) / nch, k8 = (kq + 7) / 8 * 8;
+        const int kc = nch == 1 ? j.k : ((int64_t)k8 * (nch - 1) < j.k ? k8 : kq);
         poff[q] = pos;
         for (int c = 0; c < nch; ++c) {
             const int k0 = c * kc;
@@ -902,8 +928,15 @@ inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>&
             const int gi = (int)gs.size();
             gs.push_back(g);
             ss.push_back(sg);
-            for (int a = 0; a < (j.m + BM - 1) / BM; ++a)
-                for (int b = 0; b < (j.n + BN - 1) / BN; ++b) ts.push_back(GemmTile{gi, a, b, 0});
+            const int MT = (j.m + BM - 1) / BM, NT = (j.n + BN - 1) / BN;
+            if (tune.xcd) {  // rows of tiles in groups of 8, each row's tiles 8 apart
+                for (int a0 = 0; a0 < MT; a0 += 8)
+                    for (int b = 0; b < NT; ++b)
+                        for (int a = a0; a < std::min(a0 + 8, MT); ++a) ts.push_back(GemmTile{gi, a, b, 0});
+            } else {
+                for (int a = 0; a < MT; ++a)
+                    for (int b = 0; b < NT; ++b) ts.push_back(GemmTile{gi, a, b, 0});
+            }
         }
         if (nch > 1) pos += (int64_t)nch * j.m * j.n;
         flop += 8.0 * j.m * j.n * (double)j.k;
@@ -983,11 +1016,12 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
                              dmax.as<double>(), T.as<double2>(), Rd.as<double2>(), nj);
             D.spans.close(t0, D.stream, &D.blk.orth_ms);
             D.blk.orth_calls += 1;
+            if (!refill) break;  // dependent columns stay zero: the host needs no flags
             KV_HIP(hipMemcpyAsync(hl.data(), live.get(), hl.size() * sizeof(int), hipMemcpyDeviceToHost, D.stream));
             D.sync();
             bool dead = false;
             for (int v : hl) dead = dead || v == 0;
-            if (!dead || !refill || attempt == 2) break;
+            if (!dead || attempt == 2) break;
             for (int q = 0; q < nj; ++q)
                 launch_refill_cols(D.stream, Y[q]->as<double2>(), n[q], b, live.as<int>() + (size_t)q * b,
                                    dmax.as<double>() + q, seed + 977u * (uint64_t)(pass * 3 + attempt + 1) + q);

@@ -1128,13 +1128,15 @@ void launch_hh_explicit(hipStream_t st, const double2* A, double2* V, int n, int
 // column-major matrix per block; entries of columns c >= w (past the last reflector) are 0.
 // One workgroup per block; the upper triangle of T_b lives packed in LDS (column j at
 // j (j + 1) / 2).  Each new column is a triangular matrix-vector product: row r is summed
-// by kTfSplit threads over interleaved j, partial sums combined through LDS.
+// by kTfSplit threads over interleaved j, partial sums combined through LDS.  The next
+// column of S is loaded into registers while the current column is formed, so the global
+// load is off the column-to-column chain; two barriers per column.
 constexpr int kTfactorMax = 128, kTfSplit = 4;
 __global__ __launch_bounds__(kTfactorMax * kTfSplit) void hh_tfactor_kernel(
     const double2* __restrict__ S, const double2* __restrict__ tau, double2* __restrict__ T, int bw,
     int nref) {
     __shared__ double2 tp[kTfactorMax * (kTfactorMax + 1) / 2];
-    __shared__ double2 sc[kTfactorMax];
+    __shared__ double2 sc[kTfactorMax], tl[kTfactorMax];
     __shared__ double2 part[kTfSplit][kTfactorMax];
     const int b = blockIdx.x;
     const int i0 = b * bw;
@@ -1142,9 +1144,21 @@ __global__ __launch_bounds__(kTfactorMax * kTfSplit) void hh_tfactor_kernel(
     const int r = threadIdx.x % kTfactorMax, q = threadIdx.x / kTfactorMax;
     const double2* Sb = S + (int64_t)b * bw * bw;
     double2* Tb = T + (int64_t)b * bw * bw;
-    for (int c = 0; c < w; ++c) {
-        if (q == 0 && r < c) sc[r] = Sb[(int64_t)c * bw + r];
-        __syncthreads();
+    const double2 zero2 = make_double2(0.0, 0.0);
+    // prologue: taus, column 0 (tau_0), S(:, 1) into sc and S(:, 2) into registers
+    double2 pre = zero2;
+    if (q == 0) {
+        double2 t = zero2;
+        if (r < w) t = tau[i0 + r];
+        tl[r] = t;
+        if (r == 0 && w > 0) tp[0] = t;
+        double2 s1 = zero2;
+        if (r < 1 && 1 < w) s1 = Sb[(int64_t)1 * bw + r];
+        sc[r] = s1;
+        if (r < 2 && 2 < w) pre = Sb[(int64_t)2 * bw + r];
+    }
+    __syncthreads();
+    for (int c = 1; c < w; ++c) {
         // partial of sum_{j=r}^{c-1} T(r, j) S(j, c) over j = r + q, r + q + kTfSplit, ...
         double sr = 0.0, si = 0.0;
         if (r < c) {
@@ -1158,7 +1172,7 @@ __global__ __launch_bounds__(kTfactorMax * kTfSplit) void hh_tfactor_kernel(
         part[q][r] = make_double2(sr, si);
         __syncthreads();
         if (q == 0) {
-            const double2 t = tau[i0 + c];
+            const double2 t = tl[c];
             if (r < c) {
                 double ar = 0.0, ai = 0.0;
 #pragma unroll
@@ -1170,12 +1184,18 @@ __global__ __launch_bounds__(kTfactorMax * kTfSplit) void hh_tfactor_kernel(
             } else if (r == c) {
                 tp[c * (c + 1) / 2 + c] = t;
             }
+            // column c + 1 of S for the next step, and the load of column c + 2
+            sc[r] = pre;
+            pre = zero2;
+            if (r < c + 2 && c + 2 < w) pre = Sb[(int64_t)(c + 2) * bw + r];
         }
-        __syncthreads();  // column c complete before it is read; sc and part free again
+        __syncthreads();  // column c and sc complete before they are read
     }
     for (int e = threadIdx.x; e < bw * bw; e += blockDim.x) {
         const int c = e / bw, rr = e % bw;
-        Tb[e] = (c < w && rr <= c) ? tp[c * (c + 1) / 2 + rr] : make_double2(0.0, 0.0);
+        double2 t = zero2;
+        if (c < w && rr <= c) t = tp[c * (c + 1) / 2 + rr];
+        Tb[e] = t;
     }
 }
 

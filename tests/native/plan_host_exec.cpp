@@ -4,8 +4,11 @@
 // Not part of the product: the product runs the same plans with HIP kernels.
 //
 // usage: plan_host_exec <mode> <in.bin> <out.bin>
-//   modes: heff0 heff1 heff2 (inputs L [W1 [W2]] R x), env_l2r / env_r2l (env W A),
+//   modes: heff0 heff1 heff2 heff2c (inputs L [W1 [W2]] R x; heff2c with the one-pass
+//          composite fold), env_l2r / env_r2l (env W A),
 //          product:<nc><opA><opB> (A B), krycoef:<reverse> (alpha/beta list), trunc
+#include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstdio>
 #include <cstdlib>
@@ -131,6 +134,7 @@ int main(int argc, char** argv) {
     try {
         if (mode.rfind("heff", 0) == 0) {
             int ns = mode[4] - '0';
+            const bool composite = mode == "heff2c";
             HT L = read_tensor();
             HT W1, W2;
             if (ns >= 1) W1 = read_tensor();
@@ -146,15 +150,23 @@ int main(int argc, char** argv) {
             HT t1 = alloc(H.s1.out), t2, t3, y = alloc(H.xl);
             exec_gemm(H.s1, C(L), C(xl), C(t1), 1.0);
             const HT* last = &t1;
-            if (ns >= 1) {
-                t2 = alloc(H.s2.out);
-                exec_wfold(H.s2, C(t1), C(t2));
-                last = &t2;
-            }
-            if (ns == 2) {
-                t3 = alloc(H.s3.out);
-                exec_wfold(H.s3, C(t2), C(t3));
+            if (composite) {  // the one-pass composite of both folds (FP32 operands)
+                t3 = alloc(H.s23.out);
+                // the device intermediate is not cleared: every element must be written
+                std::fill(t3.d.begin(), t3.d.end(), std::nan(""));
+                exec_wfold(H.s23, C(t1), C(t3));
                 last = &t3;
+            } else {
+                if (ns >= 1) {
+                    t2 = alloc(H.s2.out);
+                    exec_wfold(H.s2, C(t1), C(t2));
+                    last = &t2;
+                }
+                if (ns == 2) {
+                    t3 = alloc(H.s3.out);
+                    exec_wfold(H.s3, C(t2), C(t3));
+                    last = &t3;
+                }
             }
             exec_gemm(H.s4, C(*last), C(R), C(y), 1.0);
             write_tensor(fo, y);

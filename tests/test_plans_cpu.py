@@ -185,3 +185,27 @@ def test_truncation_rule_matches_oracle(harness, tmp_path):
         raw += struct.pack("d", cut) + struct.pack("i", md)
         out = list(np.frombuffer(run(harness, tmp_path, "trunc", raw=raw), dtype=np.int32))
         assert out == expect, (sv, cut, md, out)
+
+
+@pytest.mark.parametrize("layout", ["merged", "pp"])
+def test_two_site_heff_plans_match_oracle_on_every_bond(harness, tmp_path, layout):
+    """The two-site H_eff, with the two MPO folds one after the other and as the one-pass
+    composite, on every bond of an evolved chain, with environments built from both ends:
+    interior bonds have sectors the boundary bond lacks."""
+    lat, mpo, eng = evolved_state(layout, N=5, L=3, steps=3, D=16, seed=4)
+    n = len(eng.mps)
+    envl0, envr0 = eng.envL[0], eng.envR[0]
+    left = [envl0]
+    for i in range(n - 1):
+        left.append(T.env_sweep(left[-1], eng.mps[i], mpo[i], True))
+    right = [envr0]
+    for i in range(n - 1, 0, -1):
+        right.append(T.env_sweep(right[-1], eng.mps[i], mpo[i], False))
+    for i in range(n - 1):
+        envl, envr = left[i], right[n - 2 - i]
+        x2 = product(eng.mps[i], eng.mps[i + 1], 1)
+        y2 = T.apply_double(envl, x2, mpo[i], mpo[i + 1], envr)
+        scale = max(np.abs(b).max() for b in y2.blocks.values())
+        for mode in ("heff2", "heff2c"):
+            got2, _ = read_tensor(run(harness, tmp_path, mode, [envl, mpo[i], mpo[i + 1], envr, x2]))
+            assert_same(got2, y2, tol=1e-12 * max(scale, 1.0))
