@@ -206,6 +206,24 @@ int kv_tdvp_destroy(kv_tdvp* s);
  * (holstein/tdvp.h:78-141). */
 int kv_tdvp_two_site(kv_tdvp* s);
 int kv_tdvp_one_site(kv_tdvp* s);
+/* One SA-1TDVP step (TDVP::sa_site, holstein/tdvp.h:230-294, with sa_svd_impl,
+ * holstein/svd.h:443-614): the one-site step with every SVD left untruncated and enriched
+ * by new zero-weight states, their number drawn from the quasi-Gaussian tail model with
+ * betas[bond] (the reference's betaSet, 1 / the fitted width: holstein/tdvp.h:343-358; at
+ * least nsites-1 values, each finite and > 0).  Seeded redesign of the reference's
+ * std::random_device draws: the same (state, betas, seed) gives the same result.  The new
+ * states are random vectors made orthonormal to the sector's singular vectors, in the
+ * sectors with room in ascending row charge. */
+int kv_tdvp_sa_one_site(kv_tdvp* s, const double* betas, int nbetas, uint64_t seed);
+/* Starts the SVS singular-value log at path ("Total Site = N", holstein/tdvp.h:71-74);
+ * every later two-site SVD appends its kept count and log10(s_i s_{i-2} / s_{i-1}^2),
+ * i >= 2 (holstein/svd.h:358, :390-400).  KV_ERR_ARG when the file cannot be written. */
+int kv_tdvp_svs_open(kv_tdvp* s, const char* path);
+/* Host only (no device): the per-bond widths bin/nls.py fits to an SVS file (50-bin
+ * histogram per bond, least-squares fit of its quasi-Gaussian density over width > 0, 1.0
+ * for bonds with at most two values).  Writes min(nbonds, max_bonds) widths and the bond
+ * count; KV_ERR_ARG if the file cannot be read, KV_ERR_FORMAT if it is not an SVS log. */
+int kv_sa_fit_widths(const char* svs_path, double* widths, int max_bonds, int* nbonds);
 /* One global Krylov step of the whole MPS: exp(-i dt H) by a Lanczos recurrence over MPS
  * (global_main, holstein/krylov.h:206-284, with H scaled by dt as holstein/tdvp.h:318-327),
  * truncating with the config's cutoff and maxdim, then rebuilding the environments.
@@ -237,7 +255,8 @@ int kv_tdvp_bond_dims(kv_tdvp* s, int* dims);
  *   29 blk_orth_gpu_s 30 blk_orth_calls: its Cholesky QR kernels (factor + inverse)
  *   31 blk_eig_gpu_s 32 blk_eig_calls: its Rayleigh-Ritz eigensolves
  *   33 blk_sectors 34 blk_fallbacks: sectors solved by the route, and sectors it handed to
- *      the full eigensolver */
+ *      the full eigensolver
+ *   35 sa_added: states added by SA-1TDVP enrichment SVDs since the driver was created */
 int kv_tdvp_stats(kv_tdvp* s, double* out, int n);
 int kv_tdvp_reset_stats(kv_tdvp* s);
 

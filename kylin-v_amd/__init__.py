@@ -92,6 +92,9 @@ SIGNATURES = {
     "kv_tdvp_destroy": (_I, [_P]),
     "kv_tdvp_two_site": (_I, [_P]),
     "kv_tdvp_one_site": (_I, [_P]),
+    "kv_tdvp_sa_one_site": (_I, [_P, _DP, _I, ctypes.c_uint64]),
+    "kv_tdvp_svs_open": (_I, [_P, ctypes.c_char_p]),
+    "kv_sa_fit_widths": (_I, [ctypes.c_char_p, _DP, _I, _IP]),
     "kv_tdvp_global_krylov": (_I, [_P, _IP]),
     "kv_tdvp_get_site": (_I, [_P, _I, _PP]),
     "kv_tdvp_bond_dims": (_I, [_P, _IP]),
@@ -469,7 +472,7 @@ STAT_NAMES = ["Evl-2", "Evl-1", "Evl-0", "SVD", "Update", "Sweep", "heff_gpu_s",
               "heff_gemm_bytes", "heff_gemm32_gpu_s", "heff_gemm32_flop", "heff_gemm32_launches",
               "heff_gemm32_bytes", "trd_gpu_s", "trd_bytes", "trd_columns", "blk_gemm_gpu_s",
               "blk_gemm_flop", "blk_gemm_launches", "blk_gemm_bytes", "blk_orth_gpu_s", "blk_orth_calls",
-              "blk_eig_gpu_s", "blk_eig_calls", "blk_sectors", "blk_fallbacks"]
+              "blk_eig_gpu_s", "blk_eig_calls", "blk_sectors", "blk_fallbacks", "sa_added"]
 
 
 class TDVP:
@@ -500,6 +503,17 @@ class TDVP:
     def one_site(self):
         self.ctx.check(self.ctx.lib.kv_tdvp_one_site(self.h))
 
+    def sa_one_site(self, betas, seed: int):
+        """One SA-1TDVP step (TDVP::sa_site, holstein/tdvp.h:230-294) with betaSet =
+        ``betas`` (1 / fitted width per bond) and the draws of seed ``seed``."""
+        b = np.ascontiguousarray(betas, dtype=np.float64)
+        self.ctx.check(self.ctx.lib.kv_tdvp_sa_one_site(self.h, b.ctypes.data_as(_DP), len(b),
+                                                        ctypes.c_uint64(seed)))
+
+    def svs_open(self, path):
+        """Start the SVS log at ``path``; every later two-site SVD appends its record."""
+        self.ctx.check(self.ctx.lib.kv_tdvp_svs_open(self.h, os.fsencode(path)))
+
     @property
     def mps(self):
         out = []
@@ -529,13 +543,16 @@ class TDVP:
         self.ctx.check(self.ctx.lib.kv_tdvp_global_krylov(self.h, ctypes.byref(n)))
         return n.value
 
-    def total_start(self, stage, on_step=None):
+    def total_start(self, stage, on_step=None, svs_path=None, seed: int = 1):
         """Stage list [nK, n2, nSA, n1] (parse_stage, holstein/tdvp.h:19-32, stages as
-        total_start, holstein/tdvp.h:318-383).  The SA-1TDVP stage is not part of this
-        engine; nSA must be 0."""
+        total_start, holstein/tdvp.h:318-383).  The SA-1TDVP stage fits the widths to the
+        SVS log the two-site sweeps wrote to ``svs_path`` (required when nSA > 0) and runs
+        sweep k under seed + k."""
         nK, n2, nSA, n1 = stage
-        if nSA:
-            raise KVError(-1, "the SA-1TDVP stage is not part of this engine")
+        if nSA and not svs_path:
+            raise KVError(-1, "the SA-1TDVP stage needs the SVS log of the two-site sweeps")
+        if svs_path:
+            self.svs_open(svs_path)
         k = 0
         for _ in range(nK):
             self.global_krylov()
@@ -547,11 +564,32 @@ class TDVP:
             if on_step:
                 on_step(k, self)
             k += 1
+        if nSA:
+            betas = [1.0 / float("%21.15f" % w) for w in fit_widths(svs_path)]
+            for _ in range(nSA):
+                self.sa_one_site(betas, seed + k)
+                if on_step:
+                    on_step(k, self)
+                k += 1
         for _ in range(n1):
             self.one_site()
             if on_step:
                 on_step(k, self)
             k += 1
+
+
+def fit_widths(svs_path):
+    """Per-bond widths fitted to an SVS log (bin/nls.py), host only."""
+    lib = library()
+    n = ctypes.c_int()
+    rc = lib.kv_sa_fit_widths(os.fsencode(svs_path), None, 0, ctypes.byref(n))
+    if rc:
+        raise KVError(rc, "kv_sa_fit_widths failed on %s" % svs_path)
+    w = np.zeros(max(n.value, 1))
+    rc = lib.kv_sa_fit_widths(os.fsencode(svs_path), w.ctypes.data_as(_DP), n.value, ctypes.byref(n))
+    if rc:
+        raise KVError(rc, "kv_sa_fit_widths failed on %s" % svs_path)
+    return [float(x) for x in w[:n.value]]
 
 
 def parse_stage(s: str):

@@ -505,6 +505,42 @@ int kv_tdvp_get_site(kv_tdvp* s, int i, kv_tensor** out) {
     });
 }
 
+int kv_tdvp_sa_one_site(kv_tdvp* s, const double* betas, int nbetas, uint64_t seed) {
+    if (!s || !s->s || !betas || nbetas < 0) return KV_ERR_ARG;
+    return guard(s->ctx, [&] {
+        KV_REQUIRE(nbetas >= s->s->nsites() - 1, kv::ERR_ARG, "SA-1TDVP needs a fitted width for every bond");
+        std::vector<double> b(betas, betas + nbetas);
+        for (double x : b) KV_REQUIRE(std::isfinite(x) && x > 0.0, kv::ERR_ARG, "SA-1TDVP: bad inverse width");
+        s->s->one_site(&b, seed);
+    });
+}
+
+int kv_tdvp_svs_open(kv_tdvp* s, const char* path) {
+    if (!s || !s->s || !path) return KV_ERR_ARG;
+    return guard(s->ctx, [&] {
+        FILE* f = std::fopen(path, "w");
+        KV_REQUIRE(f != nullptr, kv::ERR_ARG, std::string("cannot write the SVS log ") + path);
+        std::fclose(f);
+        s->s->open_svs(path);
+    });
+}
+
+int kv_sa_fit_widths(const char* svs_path, double* widths, int max_bonds, int* nbonds) {
+    if (!svs_path || !nbonds || max_bonds < 0 || (max_bonds > 0 && !widths)) return KV_ERR_ARG;
+    FILE* f = std::fopen(svs_path, "rb");
+    if (!f) return KV_ERR_ARG;
+    std::string text;
+    char buf[65536];
+    size_t got;
+    while ((got = std::fread(buf, 1, sizeof buf, f)) > 0) text.append(buf, got);
+    std::fclose(f);
+    std::vector<double> w;
+    if (!kv::sa_fit_widths(text, w)) return KV_ERR_FORMAT;
+    *nbonds = (int)w.size();
+    for (int i = 0; i < (int)w.size() && i < max_bonds; ++i) widths[i] = w[i];
+    return KV_OK;
+}
+
 int kv_tdvp_global_krylov(kv_tdvp* s, int* iterations) {
     if (!s || !s->s) return KV_ERR_ARG;
     return guard(s->ctx, [&] {
@@ -527,7 +563,7 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
     const kv::KernelStats& e = s->s->env_kernels();
     const kv::TrdStats& r = s->s->trd_stats();
     const kv::BlockStats& bk = s->s->device().blk;
-    const double v[35] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
+    const double v[36] = {t.evl2, t.evl1, t.evl0, t.svd, t.update, t.sweep,
                           (h.gemm_ms + h.wfold_ms) * 1e-3, h.gemm_flop + h.wfold_flop,
                           (e.gemm_ms + e.wfold_ms) * 1e-3, e.gemm_flop + e.wfold_flop,
                           (double)s->s->lanczos_stats().matvecs,
@@ -538,8 +574,9 @@ int kv_tdvp_stats(kv_tdvp* s, double* out, int n) {
                           r.ms * 1e-3, r.bytes, (double)r.columns,
                           bk.gemm_ms * 1e-3, bk.gemm_flop, (double)bk.gemm_launches, bk.gemm_bytes,
                           bk.orth_ms * 1e-3, (double)bk.orth_calls, bk.eig_ms * 1e-3,
-                          (double)bk.eig_calls, (double)bk.sectors, (double)bk.fallbacks};
-    for (int i = 0; i < n && i < 35; ++i) out[i] = v[i];
+                          (double)bk.eig_calls, (double)bk.sectors, (double)bk.fallbacks,
+                          (double)s->s->sa_added()};
+    for (int i = 0; i < n && i < 36; ++i) out[i] = v[i];
     return KV_OK;
 }
 

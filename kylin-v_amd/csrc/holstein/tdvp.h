@@ -1,7 +1,8 @@
 // TDVP sweep driver and observables on the device.
 // Follows class TDVP (holstein/tdvp.h:34-386): boundary environments and the right
-// stack (:300-316), two_site (:144-227), one_site (:78-141); Operator::join
-// (holstein/operator.h:178-198) and overlap(State, State) (holstein/state.h:206-227).
+// stack (:300-316), two_site (:144-227) with its SVS log (:71-74, holstein/svd.h:390-400),
+// one_site (:78-141) and sa_site (:230-294); Operator::join (holstein/operator.h:178-198)
+// and overlap(State, State) (holstein/state.h:206-227).
 #pragma once
 
 #include <chrono>
@@ -12,6 +13,7 @@
 #include <vector>
 
 #include "exponential.h"
+#include "sa.h"
 #include "svd.h"
 
 namespace kv {
@@ -115,6 +117,7 @@ class TDVP {
             tm_.evl2 += te;
             Clock c2;
             SvdResult r = bond_svd(*mm, i, true);
+            log_svs(r);
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
@@ -142,6 +145,7 @@ class TDVP {
             tm_.evl2 += te;
             Clock c2;
             SvdResult r = bond_svd(*mm, i - 1, false);
+            log_svs(r);
             const double ts = c2.s();
             tm_.svd += ts;
             if (verbose_)
@@ -166,16 +170,28 @@ class TDVP {
         tm_.update += c5.s();
     }
 
-    // One time step of one-site TDVP (holstein/tdvp.h:78-141).
-    void one_site() {
+    // One time step of one-site TDVP (holstein/tdvp.h:78-141).  With betas (one per bond:
+    // the inverse fitted widths) it is the SA-1TDVP step (holstein/tdvp.h:230-294): every
+    // SVD is the enrichment SVD of bond i (sweeping right) or i-1 (sweeping left), the
+    // sweep's SVDs numbered 0, 1, ... as the `call` of their draws under `seed`.
+    void one_site(const std::vector<double>* betas = nullptr, uint64_t seed = 0) {
         const int ns = (int)mps_.size();
+        KV_REQUIRE(!betas || (int)betas->size() >= ns - 1, ERR_ARG,
+                   "SA-1TDVP needs a fitted width for every bond");
+        uint64_t call = 0;
+        auto split = [&](const Tensor& t, int r1, bool l2r, int bond) -> SvdResult {
+            if (!betas) return svd_truncate(dev_, *sol_, t, r1, cfg_.cutoff, cfg_.maxdim, l2r);
+            SaSvdResult e = svd_enrich(dev_, *sol_, t, r1, l2r, cfg_.cutoff, cfg_.maxdim, (*betas)[bond], seed, call++);
+            sa_added_ += e.added;
+            return std::move(e.svd);
+        };
         for (int i = 0; i < ns; ++i) {
             Clock c;
             krylov(1, mps_[i], mpo_[i].get(), nullptr, false);
             tm_.evl1 += c.s();
             if (i < ns - 1) {
                 Clock c2;
-                SvdResult r = svd_truncate(dev_, *sol_, *mps_[i], 2, cfg_.cutoff, cfg_.maxdim, true);
+                SvdResult r = split(*mps_[i], 2, true, i);
                 tm_.svd += c2.s();
                 mps_[i] = std::move(r.left);
                 Clock c3;
@@ -196,7 +212,7 @@ class TDVP {
             tm_.evl1 += c.s();
             if (i > 0) {
                 Clock c2;
-                SvdResult r = svd_truncate(dev_, *sol_, *mps_[i], 1, cfg_.cutoff, cfg_.maxdim, false);
+                SvdResult r = split(*mps_[i], 1, false, i - 1);
                 tm_.svd += c2.s();
                 mps_[i] = std::move(r.right);
                 Clock c3;
@@ -236,9 +252,24 @@ class TDVP {
         dev_->blk = BlockStats();
     }
     const TrdStats& trd_stats() const { return dev_->trd; }
+    // States added by the enrichment SVDs since construction.
+    long long sa_added() const { return sa_added_; }
+    // Starts the SVS log at path ("Total Site = N", holstein/tdvp.h:71-74); every later
+    // two-site SVD appends its record.
+    void open_svs(const std::string& path) {
+        svs_path_ = path;
+        svs_write_header(path, (int)mps_.size());
+    }
     bool timing = true;
 
   private:
+    void log_svs(const SvdResult& r) {
+        if (svs_path_.empty()) return;
+        std::vector<double> kept;
+        for (const auto& v : r.svals) kept.insert(kept.end(), v.begin(), v.end());
+        std::sort(kept.begin(), kept.end(), [](double a, double b) { return a > b; });
+        svs_append(svs_path_, kept);
+    }
     // The two-site bond SVD of bond (i, i+1) in one sweep direction (S to the right when l2r).
     SvdResult bond_svd(const Tensor& mm, int /*i*/, bool l2r) {
         return svd_truncate(dev_, *sol_, mm, 2, cfg_.cutoff, cfg_.maxdim, l2r, -1, svd_opt_);
@@ -271,6 +302,8 @@ class TDVP {
     double env_flop_ = 0;
     bool verbose_ = false;
     int last_iters_ = 0;
+    long long sa_added_ = 0;
+    std::string svs_path_;
     SvdOptions svd_opt_ = SvdOptions::from_env();
 };
 

@@ -7,8 +7,12 @@
 // After every sweep the MPS is written to State/T-<sweep>.mps in the reference's binary
 // format (holstein/tdvp.h:325-338) and "Max bond dimension = D on bond i-j" is printed
 // (holstein/state.h:187-200).  Extra flags: -p fp64|fp32|bf16x3 (H_eff contraction precision),
-// -g device.  The nK stage runs global Krylov steps (holstein/krylov.h:206-284) before
-// the sweeps, as holstein/tdvp.h:318-327; the SA-1TDVP (nSA) stage is rejected.
+// -g device, -r seed of the SA-1TDVP draws (default 1; sweep k draws under seed + k).
+// The nK stage runs global Krylov steps (holstein/krylov.h:206-284) before the sweeps, as
+// holstein/tdvp.h:318-327.  Every two-site SVD appends to ./SVS (holstein/tdvp.h:71-74,
+// holstein/svd.h:390-400); the nSA stage fits the per-bond widths to it (bin/nls.py, run
+// in-process), writes them to ./beta, reads them back as betaSet = 1 / width
+// (holstein/tdvp.h:341-358) and runs the SA-1TDVP sweeps (holstein/tdvp.h:359-373).
 #include <getopt.h>
 #include <sys/stat.h>
 
@@ -51,12 +55,14 @@ int main(int argc, char** argv) {
                                      {"tolerance", required_argument, nullptr, 't'},
                                      {"precision", required_argument, nullptr, 'p'},
                                      {"device", required_argument, nullptr, 'g'},
+                                     {"seed", required_argument, nullptr, 'r'},
                                      {"help", no_argument, nullptr, 'h'},
                                      {nullptr, 0, nullptr, 0}};
     std::string mps_file, mpo_file, stage = "[0,0,0,0]";
     double dt = 0.0, tol = 1e-8;
     int maxdim = 1000, prec = KV_FP64, device = 0, ch, idx = 0;
-    while ((ch = getopt_long(argc, argv, "w:o:s:v:D:t:p:g:h", long_op, &idx)) != -1) {
+    unsigned long long seed = 1;
+    while ((ch = getopt_long(argc, argv, "w:o:s:v:D:t:p:g:r:h", long_op, &idx)) != -1) {
         switch (ch) {
             case 'w': mps_file = optarg; break;
             case 'o': mpo_file = optarg; break;
@@ -70,10 +76,12 @@ int main(int argc, char** argv) {
                                                             : KV_FP64;
                 break;
             case 'g': device = std::atoi(optarg); break;
+            case 'r': seed = std::strtoull(optarg, nullptr, 10); break;
             case 'h':
                 std::printf("Program to calculate tDMRG (TDVP) on an MI355X\n"
                             "Usage: kln-tdvp -w [MPS file] -o [MPO file] -s [n_K,n_2,n_sa,n_1] "
-                            "-v [dt] -D [max bond] -t [SVD cutoff] [-p fp64|fp32|bf16x3] [-g device]\n");
+                            "-v [dt] -D [max bond] -t [SVD cutoff] [-p fp64|fp32|bf16x3] [-g device] "
+                            "[-r seed]\n");
                 return 0;
             default: std::printf("Unknown Option!\n"); break;
         }
@@ -81,10 +89,6 @@ int main(int argc, char** argv) {
     int st[4];
     if (!parse_stage(stage, st)) {
         std::fprintf(stderr, "kln-tdvp: bad stage string %s\n", stage.c_str());
-        return 1;
-    }
-    if (st[2] != 0) {
-        std::fprintf(stderr, "kln-tdvp: the SA-1TDVP stage is not supported\n");
         return 1;
     }
     kv_ctx* ctx = nullptr;
@@ -106,17 +110,45 @@ int main(int argc, char** argv) {
                 "SVD maximal allowed        = %d\n", 1e-8, dt, tol, maxdim);
     kv_tdvp* drv = nullptr;
     if ((rc = kv_tdvp_create(ctx, ns, mps.data(), mpo.data(), &cfg, &drv))) die(ctx, rc, "TDVP setup");
+    if ((rc = kv_tdvp_svs_open(drv, "SVS"))) die(ctx, rc, "opening the SVS log");
     mkdir("State", 0755);
     std::vector<int> bonds(ns > 1 ? ns - 1 : 1);
+    std::vector<double> betas;
     int swp = 0;
-    const char* stage_name[3] = {"Krylov", "2TDVP", "1TDVP"};
-    for (int stage_i = 0; stage_i < 3; ++stage_i) {
-        const int n = stage_i == 0 ? st[0] : stage_i == 1 ? st[1] : st[3];
+    const char* stage_name[4] = {"Krylov", "2TDVP", "SA-1TDVP", "1TDVP"};
+    for (int stage_i = 0; stage_i < 4; ++stage_i) {
+        const int n = st[stage_i];
+        if (stage_i == 2) {
+            if (n == 0) continue;
+            int nb = 0;
+            if ((rc = kv_sa_fit_widths("SVS", nullptr, 0, &nb))) die(ctx, rc, "fitting the SVS widths");
+            std::vector<double> w(nb > 0 ? nb : 1);
+            if ((rc = kv_sa_fit_widths("SVS", w.data(), nb, &nb))) die(ctx, rc, "fitting the SVS widths");
+            FILE* fb = std::fopen("beta", "w");
+            if (!fb) die(ctx, KV_ERR_ARG, "writing the beta file");
+            for (int b = 0; b < nb; ++b) std::fprintf(fb, "%21.15f\n", w[b]);
+            std::fclose(fb);
+            std::printf("Fitting OK!\n");
+            fb = std::fopen("beta", "r");
+            if (!fb) die(ctx, KV_ERR_ARG, "reading the beta file");
+            char line[256];
+            while (std::fgets(line, sizeof line, fb))
+                if (std::strlen(line) > 4) betas.push_back(1.0 / std::atof(line));
+            std::fclose(fb);
+            std::printf("Read betas OK!\n");
+            if ((int)betas.size() < ns - 1) {
+                std::fprintf(stderr, "kln-tdvp: the SA-1TDVP stage needs a width for each of the %d bonds "
+                             "(%d fitted: run two-site sweeps first)\n", ns - 1, (int)betas.size());
+                return 1;
+            }
+        }
         std::printf("\n ---------------- Start %s stage ----------------\n", stage_name[stage_i]);
         for (int k = 0; k < n; ++k, ++swp) {
             if (stage_i > 0) std::printf("\nSweep %d:\n", swp + 1);
-            rc = stage_i == 0 ? kv_tdvp_global_krylov(drv, nullptr)
-                 : stage_i == 1 ? kv_tdvp_two_site(drv) : kv_tdvp_one_site(drv);
+            rc = stage_i == 0   ? kv_tdvp_global_krylov(drv, nullptr)
+                 : stage_i == 1 ? kv_tdvp_two_site(drv)
+                 : stage_i == 2 ? kv_tdvp_sa_one_site(drv, betas.data(), (int)betas.size(), seed + swp)
+                                : kv_tdvp_one_site(drv);
             if (rc) die(ctx, rc, "sweep");
             kv_tdvp_bond_dims(drv, bonds.data());
             int mb = 0, mbp = 0;

@@ -13,6 +13,7 @@ Reference semantics followed (file:line in YiHeXu517/Kylin-V):
   local_krylov           holstein/exponential.h:92-190
   svd_impl               holstein/svd.h:296-440
   TDVP one_site/two_site holstein/tdvp.h:78-227, total_start init :297-316
+  SA-1TDVP sweep         holstein/tdvp.h:230-294 (enrichment SVD in oracle/sa.py)
   Operator::join         holstein/operator.h:178-198
   overlap(State, State)  holstein/state.h:206-227
 """
@@ -320,6 +321,8 @@ class TDVP:
         self.tol, self.max_iter = tol_krylov, max_iter
         self.krylov_log = []
         self.kept_log = []
+        self.svs = []  # (kept count, SVS record) per two-site SVD (holstein/svd.h:390-400)
+        self.svs_values = []  # the kept values behind each record, descending
         envl, envr = boundary_envs(self.mps)
         self.envL = [envl]
         self.envR = [envr]
@@ -339,7 +342,8 @@ class TDVP:
         for i in range(ns - 1):
             mm = product(self.mps[i], self.mps[i + 1], 1)
             mm = self._kry(mm, self.mpo[i], self.mpo[i + 1], False, 2)
-            cur, nex = svd_impl(mm, 2, self.cutoff, self.maxdim, True)
+            cur, nex, sv = svd_impl(mm, 2, self.cutoff, self.maxdim, True, return_svals=True)
+            self._log_svs(sv)
             self.kept_log.append(sum(d for d in cur.legs()[2].values()))
             self.mps[i] = cur
             if i < ns - 2:
@@ -352,7 +356,8 @@ class TDVP:
         for i in range(ns - 1, 0, -1):
             mm = product(self.mps[i - 1], self.mps[i], 1)
             mm = self._kry(mm, self.mpo[i - 1], self.mpo[i], False, 2)
-            nex, cur = svd_impl(mm, 2, self.cutoff, self.maxdim, False)
+            nex, cur, sv = svd_impl(mm, 2, self.cutoff, self.maxdim, False, return_svals=True)
+            self._log_svs(sv)
             self.kept_log.append(sum(d for d in nex.legs()[2].values()))
             self.mps[i] = cur
             if i > 1:
@@ -364,13 +369,32 @@ class TDVP:
                 self.mps[i - 1] = nex
         self.envR.append(env_sweep(self.envR[-1], self.mps[1], self.mpo[1], False))
 
-    def one_site(self):
-        """One time step dt of one-site TDVP (tdvp.h:78-141)."""
+    def _log_svs(self, sv):
+        from .sa import svs_record
+        kept = sorted((float(x) for v in sv.values() for x in v), reverse=True)
+        self.svs.append((len(kept), svs_record(kept)))
+        self.svs_values.append(kept)
+
+    def one_site(self, betas=None, seed: int = 0):
+        """One time step dt of one-site TDVP (tdvp.h:78-141); with ``betas`` (one per bond,
+        the inverse fitted widths) the SA-1TDVP step (tdvp.h:230-294): every SVD is the
+        enrichment SVD of bond i (sweeping right) or i-1 (sweeping left), numbered 0, 1, ...
+        in sweep order as the ``call`` of its draws under ``seed``."""
+        from .sa import sa_svd_impl
         ns = len(self.mps)
+        calls = [0]
+
+        def split(t, r1, l2r, bond):
+            if betas is None:
+                return svd_impl(t, r1, self.cutoff, self.maxdim, l2r)
+            lef, rig, _ = sa_svd_impl(t, r1, self.cutoff, self.maxdim, l2r, betas[bond], seed, calls[0])
+            calls[0] += 1
+            return lef, rig
+
         for i in range(ns):
             self.mps[i] = self._kry(self.mps[i], self.mpo[i], None, False, 1)
             if i < ns - 1:
-                cur, nex = svd_impl(self.mps[i], 2, self.cutoff, self.maxdim, True)
+                cur, nex = split(self.mps[i], 2, True, i)
                 self.mps[i] = cur
                 self.envL.append(env_sweep(self.envL[-1], self.mps[i], self.mpo[i], True))
                 nex = self._kry(nex, None, None, True, 0)
@@ -379,7 +403,7 @@ class TDVP:
         for i in range(ns - 1, -1, -1):
             self.mps[i] = self._kry(self.mps[i], self.mpo[i], None, False, 1)
             if i > 0:
-                nex, cur = svd_impl(self.mps[i], 1, self.cutoff, self.maxdim, False)
+                nex, cur = split(self.mps[i], 1, False, i - 1)
                 self.mps[i] = cur
                 self.envR.append(env_sweep(self.envR[-1], self.mps[i], self.mpo[i], False))
                 nex = self._kry(nex, None, None, True, 0)
