@@ -173,13 +173,13 @@ int kv_krylov_step(kv_ctx* ctx, int nsite, const kv_tensor* L, const kv_tensor* 
 /* svd_impl<R1,R2>(A, cfg, LeftDir=-1) (holstein/svd.h:296-440): sector SVD in FP64,
  * global descending sort, drop entry i (never 0) while i >= maxdim or s_i <= cutoff;
  * S absorbed into V^dag if left_to_right else into U.  nkept receives the kept bond
- * dimension.  Sectors whose smaller side is at least twice maxdim + 32 take a block
+ * dimension.  Sectors whose smaller side is at least three times maxdim + 32 take a block
  * subspace route: the isometric side's leading singular subspace from a seeded random
  * block of maxdim + 32 columns with power steps, Cholesky QR between the products and
  * Rayleigh-Ritz, each Ritz pair's residual checked against 1e-13 of the largest value;
  * a cut the certified values cannot settle sends the sector to the full route.  Otherwise
  * sectors whose smaller side is at least 64 are solved through their Gram matrix on the
- * isometric side (Hermitian eigen-decomposition: from side 1024, or for sectors of
+ * isometric side (Hermitian eigen-decomposition: from side 512, or for sectors of
  * different sides, the engine's batched Householder tridiagonalisation, rocSOLVER's dstedc
  * and the kept vectors carried back through the reflectors; below that rocSOLVER's zheevd;
  * a second pass over the tail below 1e-5 of the largest value); smaller ones by rocSOLVER's

@@ -83,14 +83,29 @@ struct SvdResult {
 // Side from which the split eigensolver replaces zheevd by default.  Below it the
 // back-transformation of all vectors inside zheevd is cheap, and the split route's extra
 // launches and host synchronisations cost more than they save: with the split route at
-// every side, 8 concurrent chains of the batching workload ran 7% slower.
-constexpr int kSplitEigMin = 1024;
+// every side, 8 concurrent chains of the batching workload ran 7% slower.  From 512 up the
+// split route wins: rocSOLVER's column-by-column reduction dominates zheevd there (the
+// 32-site chain's edge bonds, 512-wide Gram matrices, and the batching chains' largest
+// sectors).
+constexpr int kSplitEigMin = 512;
+// The side from which the split route is taken (KV_SPLIT_MIN overrides kSplitEigMin).
+inline int split_eig_min() {
+    static const int v = [] {
+        const char* e = std::getenv("KV_SPLIT_MIN");
+        const int x = e ? std::atoi(e) : 0;
+        return x >= 2 ? x : kSplitEigMin;
+    }();
+    return v;
+}
 enum EigMode { EIG_AUTO = 0, EIG_SPLIT = 1, EIG_HEEVD = 2 };
 
 struct SvdOptions {
     bool block = true;      // block subspace route for sectors far larger than the kept dimension
     double guard = 1e-4;    // smaller-side Gram vectors only if s_kept_min >= guard * s_max
     int iso_min = 64;       // smaller side from which the Gram routes are used
+    double block_min = 3.0; // block route for sectors whose smaller side is >= block_min x the block
+                            // (at 2x, 512 x 2048 sectors keeping half their values exhausted
+                            // the power steps and fell back after the work was spent)
     double tail = 1e-5;     // isometric route: values below tail * s_max get a second pass
     bool force_exact = false;
     int eig_mode = EIG_AUTO;  // Gram eigensolver: split from kSplitEigMin, or forced
@@ -108,6 +123,7 @@ struct SvdOptions {
             o.eig_mode = std::string(e) == "heevd" ? EIG_HEEVD : std::string(e) == "split" ? EIG_SPLIT : EIG_AUTO;
         if (const char* g = std::getenv("KV_GRAM_GUARD")) o.guard = std::atof(g);
         if (const char* m = std::getenv("KV_ISO_MIN")) o.iso_min = std::atoi(m);
+        if (const char* m = std::getenv("KV_BLOCK_MIN")) o.block_min = std::atof(m);
         if (const char* t = std::getenv("KV_ISO_TAIL")) o.tail = std::atof(t);
         return o;
     }
@@ -382,7 +398,7 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
     }
     const size_t NN = (size_t)N * N;
     const bool split = mode == EIG_SPLIT ? N >= 2
-                       : mode == EIG_AUTO ? (N >= kSplitEigMin || (!same && N >= 2))
+                       : mode == EIG_AUTO ? (N >= split_eig_min() || (!same && N >= 2))
                                           : false;
     KV_REQUIRE(split || same, ERR_INTERNAL, "zheevd needs sectors of one side");
     DevBuf W(D.pool.get(), (size_t)N * nb * sizeof(double)), E(D.pool.get(), (size_t)N * nb * sizeof(double));
@@ -1297,8 +1313,8 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             launch_copy2d<double2>(D.stream, A.d(), X[p].M.as<double2>(), dc.as<Copy2D>(),
                                    (int)cps.size(), maxrows, nullptr);
         }
-        // sectors at least twice as wide as the block take the block subspace route
-        if (block_ok && std::min(rs.len, cs.len) >= 2 * (int64_t)bw)
+        // sectors at least block_min (3) times as wide as the block take the block subspace route
+        if (block_ok && (double)std::min(rs.len, cs.len) >= opt.block_min * bw)
             cand.push_back(p);
         else
             full_route(p);
