@@ -99,15 +99,15 @@ inline uint64_t gemm_shape_key(const ProductPlan& p, int prec) {
 
 struct DevProduct {
     ProductPlan p;
-    DevBuf groups, segs;
-    DevBuf tiles[kNumGemmConfigs];
+    PackedUpload lists;  // groups, segments and config 0's tiles, one upload
+    DevBuf tiles[kNumGemmConfigs];  // other configs' tiles, cut on first use
+    const GemmTile* tp[kNumGemmConfigs] = {nullptr};
     int ntiles[kNumGemmConfigs] = {0};
     bool cut[kNumGemmConfigs] = {false};
     void build(Device& dev, ProductPlan plan) {
         p = std::move(plan);
-        groups = upload_vec(dev, p.groups);
-        segs = upload_vec(dev, p.segs);
-        tiles[0] = upload_vec(dev, p.tiles);
+        lists = upload_packed(dev, p.groups, p.segs, p.tiles);
+        tp[0] = lists.at<GemmTile>(2);
         ntiles[0] = (int)p.tiles.size();
         cut[0] = true;
     }
@@ -115,14 +115,15 @@ struct DevProduct {
         if (cut[cfg]) return;
         std::vector<GemmTile> t = cut_tiles(p.groups, cfg);
         tiles[cfg] = upload_vec(dev, t);
+        tp[cfg] = tiles[cfg].as<GemmTile>();
         ntiles[cfg] = (int)t.size();
         cut[cfg] = true;
     }
     template <class T2>
     void launch(Device& dev, int cfg, const T2* A, const T2* B, T2* C, double ar, double ai) {
         ensure_tiles(dev, cfg);
-        launch_gemm<T2>(dev.stream, p.opA, p.opB, A, B, C, tiles[cfg].as<GemmTile>(), ntiles[cfg],
-                        groups.as<GemmGroup>(), segs.as<GemmSeg>(), ar, ai, cfg);
+        launch_gemm<T2>(dev.stream, p.opA, p.opB, A, B, C, tp[cfg], ntiles[cfg], lists.at<GemmGroup>(0),
+                        lists.at<GemmSeg>(1), ar, ai, cfg);
     }
     // Tile config for this plan: cached per shape class; on a first encounter every config
     // built for the op pair runs twice on the real operands and the fastest is kept.
@@ -164,8 +165,8 @@ struct DevProduct {
     void launch_split(Device& dev, int cfg, const float2* A, const float2* B, float2* C, double ar,
                       double ai) {
         ensure_tiles(dev, cfg);
-        launch_gemm_bf16x3(dev.stream, p.opA, p.opB, A, B, C, tiles[cfg].as<GemmTile>(),
-                           ntiles[cfg], groups.as<GemmGroup>(), segs.as<GemmSeg>(), ar, ai, cfg);
+        launch_gemm_bf16x3(dev.stream, p.opA, p.opB, A, B, C, tp[cfg], ntiles[cfg], lists.at<GemmGroup>(0),
+                           lists.at<GemmSeg>(1), ar, ai, cfg);
     }
     // Split-BF16 tile (64x64 or 128x128), timed once per shape class like choose().
     int choose_split(Device& dev, const float2* A, const float2* B, float2* C, double ar, double ai) {
@@ -226,11 +227,10 @@ struct DevProduct {
 
 struct DevWFold {
     WFoldPlan p;
-    DevBuf tasks, terms;
+    PackedUpload lists;  // tasks and terms, one upload
     void build(Device& dev, WFoldPlan plan) {
         p = std::move(plan);
-        tasks = upload_vec(dev, p.tasks);
-        terms = upload_vec(dev, p.terms);
+        lists = upload_packed(dev, p.tasks, p.terms);
     }
     template <class T2>
     void run(Device& dev, const T2* in, T2* out, EventLog* log = nullptr) const {
@@ -240,8 +240,8 @@ struct DevWFold {
             b = log->get();
             KV_HIP(hipEventRecord(a, dev.stream));
         }
-        launch_wfold<T2>(dev.stream, in, out, tasks.as<WTask>(), (int)p.tasks.size(),
-                         terms.as<WTerm>(), p.maxPQ);
+        launch_wfold<T2>(dev.stream, in, out, lists.at<WTask>(0), (int)p.tasks.size(),
+                         lists.at<WTerm>(1), p.maxPQ);
         if (log) {
             KV_HIP(hipEventRecord(b, dev.stream));
             log->pending.push_back({a, b, p.flop, 1, 0.0});

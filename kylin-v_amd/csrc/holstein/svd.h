@@ -150,9 +150,9 @@ inline void gemm_dense(Device& D, char opA, char opB, const double2* A, int lda,
     for (int a = 0; a < (m + GEMM_BM - 1) / GEMM_BM; ++a)
         for (int b = 0; b < (n + GEMM_BN - 1) / GEMM_BN; ++b)
             if (!upper_only || a * GEMM_BM < (b + 1) * GEMM_BN) ts.push_back(GemmTile{0, a, b, 0});
-    DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
-    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, C, dt.as<GemmTile>(),
-                         (int)ts.size(), dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0);
+    PackedUpload pk = upload_packed(D, gs, ss, ts);
+    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), A, B, C, pk.at<GemmTile>(2),
+                         (int)ts.size(), pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0, 0.0);
 }
 
 struct SectorSvd {
@@ -637,11 +637,11 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
                     std::min(kc, mb - c * kc));
             y_cnt[b] = (int)ts.size() - y_off[b];
         }
-        DevBuf dg = upload_vec(D, gs), dsg = upload_vec(D, ss), dt = upload_vec(D, ts);
-        const GemmTile* tiles = dt.as<GemmTile>();
+        PackedUpload pk = upload_packed(D, gs, ss, ts);
+        const GemmTile* tiles = pk.at<GemmTile>(2);
         launch_zero(D.stream, Sp.get(), Sp.bytes());  // blocks with fewer chunks leave parts unset
         launch_gemm<double2>(D.stream, 0, 2, V.as<double2>(), V.as<double2>(), Sp.as<double2>(), tiles,
-                             s_tiles, dg.as<GemmGroup>(), dsg.as<GemmSeg>(), 1.0, 0.0);
+                             s_tiles, pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0, 0.0);
         launch_sum_parts(D.stream, Sp.as<double2>(), nch_max, (int64_t)nblk * sbw, S.as<double2>());
         launch_hh_tfactor(D.stream, S.as<double2>(), X.tau.as<double2>(), T.as<double2>(), nblk, bw, nref);
         auto* v = reinterpret_cast<rocblas_double_complex*>(V.get());
@@ -663,7 +663,7 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
             const int mb = n - r0;
             if (mb <= 0) continue;
             launch_gemm<double2>(D.stream, 0, 2, Z.as<double2>(), V.as<double2>(), Yp.as<double2>(),
-                                 tiles + y_off[b], y_cnt[b], dg.as<GemmGroup>(), dsg.as<GemmSeg>(), 1.0,
+                                 tiles + y_off[b], y_cnt[b], pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0,
                                  0.0);
             launch_sum_parts(D.stream, Yp.as<double2>(), nch[b], (int64_t)bw * k, Y.as<double2>());
             ok(rocblas_zgemm(sol.h, rocblas_operation_none, rocblas_operation_none, mb, k, bw, &mone,
@@ -958,10 +958,10 @@ inline void gemm_jobs(Device& D, char opA, char opB, const std::vector<GemmJob>&
         flop += 8.0 * j.m * j.n * (double)j.k;
         bytes += 16.0 * ((double)j.m * j.k + (double)j.k * j.n + (double)j.m * j.n);
     }
-    DevBuf dg = upload_vec(D, gs), ds = upload_vec(D, ss), dt = upload_vec(D, ts);
+    PackedUpload pk = upload_packed(D, gs, ss, ts);
     hipEvent_t t0 = D.spans.mark(D.stream);
-    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), a0, b0, c0, dt.as<GemmTile>(), (int)ts.size(),
-                         dg.as<GemmGroup>(), ds.as<GemmSeg>(), 1.0, 0.0, cfg);
+    launch_gemm<double2>(D.stream, op_code(opA), op_code(opB), a0, b0, c0, pk.at<GemmTile>(2), (int)ts.size(),
+                         pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0, 0.0, cfg);
     if (nch > 1)
         for (size_t q = 0; q < jobs.size(); ++q)
             launch_sum_parts(D.stream, part.as<double2>() + poff[q], nch, (int64_t)jobs[q].m * jobs[q].n,
@@ -1560,10 +1560,10 @@ inline SvdResult svd_truncate(const std::shared_ptr<Device>& dev, Solver& sol, c
             maxr_r = std::max(maxr_r, c.rows);
             rc.push_back(c);
         }
-        DevBuf dl = upload_vec(D, lc), dr = upload_vec(D, rc);
-        launch_copy2d<double2>(D.stream, X[p].U.as<double2>(), res.left->d(), dl.as<Copy2D>(),
+        PackedUpload pk = upload_packed(D, lc, rc);
+        launch_copy2d<double2>(D.stream, X[p].U.as<double2>(), res.left->d(), pk.at<Copy2D>(0),
                                (int)lc.size(), maxr_l, X[p].S.as<double>());
-        launch_copy2d<double2>(D.stream, X[p].Vh.as<double2>(), res.right->d(), dr.as<Copy2D>(),
+        launch_copy2d<double2>(D.stream, X[p].Vh.as<double2>(), res.right->d(), pk.at<Copy2D>(1),
                                (int)rc.size(), maxr_r, X[p].S.as<double>());
         res.svals[p].assign(X[p].s.begin(), X[p].s.begin() + nk);
     }

@@ -3,7 +3,9 @@
 // holstein/tensor.h:16-212), whose blocks are separate host mallocs.
 #pragma once
 
+#include <cstring>
 #include <memory>
+#include <type_traits>
 #include <vector>
 
 #include "../kv_kernels.h"
@@ -76,6 +78,41 @@ inline DevBuf upload_vec(Device& dev, const std::vector<T>& v) {
     DevBuf b(dev.pool.get(), v.size() * sizeof(T));
     dev.upload(b.get(), v.data(), v.size() * sizeof(T));
     return b;
+}
+
+// Several host arrays in one device buffer by one upload (each array 256-byte aligned),
+// so a launch's descriptor lists cost one copy instead of one each.
+struct PackedUpload {
+    DevBuf buf;
+    std::vector<size_t> off;
+    template <class T>
+    T* at(int i) const {
+        return reinterpret_cast<T*>(static_cast<char*>(buf.get()) + off[i]);
+    }
+};
+
+template <class... V>
+inline PackedUpload upload_packed(Device& dev, const V&... vs) {
+    PackedUpload p;
+    size_t total = 0;
+    auto place = [&](const auto& v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        const size_t o = (total + 255) / 256 * 256;
+        p.off.push_back(o);
+        total = o + v.size() * sizeof(T);
+    };
+    (place(vs), ...);
+    std::vector<char> h(total);
+    int i = 0;
+    auto fill = [&](const auto& v) {
+        using T = typename std::decay_t<decltype(v)>::value_type;
+        if (!v.empty()) std::memcpy(h.data() + p.off[i], v.data(), v.size() * sizeof(T));
+        ++i;
+    };
+    (fill(vs), ...);
+    p.buf = DevBuf(dev.pool.get(), total);
+    dev.upload(p.buf.get(), h.data(), total);
+    return p;
 }
 
 }  // namespace kv
