@@ -1583,7 +1583,7 @@ __global__ __launch_bounds__(256) void trd_matvec_kernel(const double2* __restri
 }
 
 // Householder tridiagonalisation in zhetrd('L')'s form of Hermitian matrices up to
-// kTrdSmallMax wide, one 512-thread workgroup per matrix (column-major, both triangles
+// kTrdSmallMax wide, one 768-thread workgroup per matrix (column-major, both triangles
 // valid, as herm_fill leaves them).  Unblocked, with each step's rank-2 update fused into the
 // next step's matrix-vector product: after column j's reflector v_j and y_j = A v_j,
 //   w_j = tau_j y_j - (tau_j / 2) (tau_j y_j)^H v_j v_j,
@@ -1593,7 +1593,7 @@ __global__ __launch_bounds__(256) void trd_matvec_kernel(const double2* __restri
 //   one pass over A(j+2:, j+2:) applying the same update and forming y_{j+1} = A v_{j+1}
 //   (as a combination of columns, exact for the Hermitian A the updates keep).
 // D, E, tau and the stored reflectors are hetrd_lower's (the back-transformation reads them).
-constexpr int kTrdSmallThreads = 512;
+constexpr int kTrdSmallThreads = 768;
 __global__ __launch_bounds__(kTrdSmallThreads) void trd_small_kernel(double2* __restrict__ A, int n,
                                                                      double* __restrict__ Dg, double* __restrict__ Eg,
                                                                      double2* __restrict__ tau) {
@@ -1657,7 +1657,9 @@ __global__ __launch_bounds__(kTrdSmallThreads) void trd_small_kernel(double2* __
     // s + wv CG + g + NW CG t; the pending update (vp, w0) is applied to A(s:, s:) first and
     // written back when `upd`.  The waves' partial vectors are summed in wave order.
     auto pass = [&](int s, const double2* v, const double2* vp, bool upd) {
-        constexpr int CG = 4;  // columns per wave whose loads are issued before the first store
+        // columns per wave whose loads are issued before the first store (2 at 12 waves: 168 VGPRs,
+        // no spills; 4 at 8 waves)
+        constexpr int CG = kTrdSmallThreads > 512 ? 2 : 4;
         double2 acc[RU];
 #pragma unroll
         for (int u = 0; u < RU; ++u) acc[u] = zero2;
