@@ -55,6 +55,11 @@ inline std::mutex& solver_init_mutex() {
 
 struct Solver {
     rocblas_handle h = nullptr;
+    // A second handle on a stream of its own (default priority), made on first use: the
+    // tridiagonal eigensolves of a block-route bond's two sectors run side by side.
+    rocblas_handle h2 = nullptr;
+    hipStream_t s2 = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     explicit Solver(Device& dev) {
         std::lock_guard<std::mutex> g(solver_init_mutex());
         if (rocblas_create_handle(&h) != rocblas_status_success)
@@ -62,8 +67,24 @@ struct Solver {
         if (rocblas_set_stream(h, dev.stream) != rocblas_status_success)
             throw Error(ERR_SOLVER, "rocblas_set_stream failed");
     }
+    rocblas_handle second() {
+        if (h2) return h2;
+        std::lock_guard<std::mutex> g(solver_init_mutex());
+        KV_HIP(hipStreamCreateWithFlags(&s2, hipStreamNonBlocking));
+        KV_HIP(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+        KV_HIP(hipEventCreateWithFlags(&join, hipEventDisableTiming));
+        if (rocblas_create_handle(&h2) != rocblas_status_success)
+            throw Error(ERR_SOLVER, "rocblas_create_handle failed");
+        if (rocblas_set_stream(h2, s2) != rocblas_status_success)
+            throw Error(ERR_SOLVER, "rocblas_set_stream failed");
+        return h2;
+    }
     ~Solver() {
         if (h) rocblas_destroy_handle(h);
+        if (h2) rocblas_destroy_handle(h2);
+        if (fork) (void)hipEventDestroy(fork);
+        if (join) (void)hipEventDestroy(join);
+        if (s2) (void)hipStreamDestroy(s2);
     }
     Solver(const Solver&) = delete;
     Solver& operator=(const Solver&) = delete;
@@ -388,7 +409,7 @@ inline void hetrd_lower(Device& D, Solver& sol, double2* A, int N, int batch, do
 // divide and conquer returns NaN eigenvectors.  dstedc of a pair on two streams measured
 // slower in the sweep than one after the other.)
 inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int mode = EIG_AUTO,
-                       bool roc_trd = false) {
+                       bool roc_trd = false, bool overlap = false) {
     const int nb = (int)xs.size();
     int N = 0;
     bool same = true;
@@ -480,13 +501,25 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
         DevBuf W0(D.pool.get(), W.bytes()), E0(D.pool.get(), E.bytes());
         KV_HIP(hipMemcpyAsync(W0.get(), W.get(), W.bytes(), hipMemcpyDeviceToDevice, D.stream));
         KV_HIP(hipMemcpyAsync(E0.get(), E.get(), E.bytes(), hipMemcpyDeviceToDevice, D.stream));
+        // with `overlap`, a pair of sectors is solved on two streams (the second handle's
+        // stream forks after everything queued so far and the context stream joins it)
+        const bool two = overlap && nb == 2;
+        if (two) {
+            sol.second();
+            KV_HIP(hipEventRecord(sol.fork, D.stream));
+            KV_HIP(hipStreamWaitEvent(sol.s2, sol.fork, 0));
+        }
+        for (int b = 0; b < nb; ++b) xs[b]->Zt = DevBuf(D.pool.get(), (size_t)xs[b]->k * xs[b]->k * sizeof(double));
         for (int b = 0; b < nb; ++b) {
             const int n = xs[b]->k;
-            xs[b]->Zt = DevBuf(D.pool.get(), (size_t)n * n * sizeof(double));
             const rocblas_status s2 = rocsolver_dstedc(
-                sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * N,
+                two && b == 1 ? sol.h2 : sol.h, rocblas_evect_tridiagonal, n, W.as<double>() + (size_t)b * N,
                 E.as<double>() + (size_t)b * N, xs[b]->Zt.as<double>(), n, info.as<rocblas_int>() + b);
             KV_REQUIRE(s2 == rocblas_status_success, ERR_SOLVER, "rocsolver_dstedc failed");
+        }
+        if (two) {
+            KV_HIP(hipEventRecord(sol.join, sol.s2));
+            KV_HIP(hipStreamWaitEvent(D.stream, sol.join, 0));
         }
         // info and the sum of squares of every sector's eigenvectors (NaN check), one wait
         std::vector<double> zsq(nb, 0.0);
@@ -1140,7 +1173,7 @@ inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool i
         hipEvent_t t0 = D.spans.mark(D.stream);
         // the split route (one-workgroup tridiagonalisation, dstedc, back-transformation):
         // rocSOLVER's zheevd spends most of its time reducing a 256-wide matrix column by column
-        gram_solve(D, sol, hp, mode == EIG_HEEVD ? EIG_HEEVD : EIG_SPLIT, roc_trd);
+        gram_solve(D, sol, hp, mode == EIG_HEEVD ? EIG_HEEVD : EIG_SPLIT, roc_trd, true);
         for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
         D.spans.close(t0, D.stream, &D.blk.eig_ms);
         D.blk.eig_calls += 1;
