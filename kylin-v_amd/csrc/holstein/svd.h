@@ -1065,7 +1065,9 @@ inline void block_orth(Device& D, std::vector<DevBuf*>& Y, const std::vector<int
                              dmax.as<double>(), T.as<double2>(), Rd.as<double2>(), nj);
             D.spans.close(t0, D.stream, &D.blk.orth_ms);
             D.blk.orth_calls += 1;
-            if (!refill) break;  // dependent columns stay zero: the host needs no flags
+            // dependent columns stay zero: the host needs no flags.  A second pass factors the
+            // Gram matrix of an orthonormal basis (pivots near one), so only the first checks.
+            if (!refill || pass > 0) break;
             KV_HIP(hipMemcpyAsync(hl.data(), live.get(), hl.size() * sizeof(int), hipMemcpyDeviceToHost, D.stream));
             D.sync();
             bool dead = false;

@@ -111,13 +111,14 @@ def chol_left(S, nb=32, tau=DROP):
 
 def block_orth(Y, passes=2, refill=False, seed=99):
     """Cholesky QR passes; dependent columns come out zero, or with `refill` are replaced by
-    random vectors at the largest column's scale and the pass repeats (at most twice)."""
+    random vectors at the largest column's scale and the first pass repeats (at most twice;
+    later passes factor an orthonormal basis and are not checked)."""
     rng = np.random.default_rng(seed)
-    for _ in range(passes):
+    for pas in range(passes):
         for attempt in range(3):
             S = Y.conj().T @ Y
             R, live, T = chol_left(S)
-            if live.all() or not refill or attempt == 2:
+            if live.all() or not refill or pas > 0 or attempt == 2:
                 break
             scale = np.sqrt(1.5 * np.real(np.diag(S)).max() / Y.shape[0])
             dead = ~live
