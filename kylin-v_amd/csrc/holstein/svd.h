@@ -604,13 +604,18 @@ inline void gram_solve(Device& D, Solver& sol, std::vector<SectorSvd*> xs, int m
 // S_b = V_b^H V_b and Y_b have few outputs (bw x bw, bw x k) over up to n rows, so both run
 // split over row chunks on the grouped GEMM (one group per block and chunk, partial
 // products summed by a second kernel); P_b and the update Z -= P_b Y_b are rocBLAS GEMMs.
-inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
+// With st and h (a second stream and the rocBLAS handle bound to it) the work goes to that
+// stream, and the temporaries move into *keep, to be released once the caller has joined it.
+inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo, hipStream_t st = nullptr,
+                         rocblas_handle h = nullptr, std::vector<DevBuf>* keep = nullptr) {
+    if (!st) st = D.stream;
+    if (!h) h = sol.h;
     const int n = X.k;
     if (X.ev_lo <= lo) return;
     KV_REQUIRE(X.Zt.get() != nullptr, ERR_INTERNAL, "Gram eigenvectors requested before the eigensolver");
     const int k = n - lo;
     DevBuf Z(D.pool.get(), (size_t)n * k * sizeof(double2));
-    launch_real_cols(D.stream, X.Zt.as<double>(), n, lo, k, n, Z.as<double2>());
+    launch_real_cols(st, X.Zt.as<double>(), n, lo, k, n, Z.as<double2>());
     const int nref = n - 1;
     if (nref > 0 && k > 0) {
         const int bw = kReflectorBlock, kc = kReflectorBlock;  // block width, row chunk
@@ -625,7 +630,7 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
         DevBuf Y(D.pool.get(), (size_t)bw * k * sizeof(double2));
         DevBuf Sp(D.pool.get(), (size_t)nch_max * nblk * sbw * sizeof(double2));
         DevBuf Yp(D.pool.get(), (size_t)nch_max * bw * k * sizeof(double2));
-        launch_hh_explicit(D.stream, X.G.as<double2>(), V.as<double2>(), n, (int)vcols);
+        launch_hh_explicit(st, X.G.as<double2>(), V.as<double2>(), n, (int)vcols);
         // work lists: S chunks of every block first, then the Y chunks block by block.
         // Row-major view: S_b^T = A B with A[c][i] = V[i, c] ('N', ld n) and B = conj(V) ('C');
         // Y_b^T = Zr conj(V_b) with Zr[j][i] = Z[i, j] ('N', ld n).
@@ -670,13 +675,13 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
                     std::min(kc, mb - c * kc));
             y_cnt[b] = (int)ts.size() - y_off[b];
         }
-        PackedUpload pk = upload_packed(D, gs, ss, ts);
+        PackedUpload pk = upload_packed_on(D, st, gs, ss, ts);
         const GemmTile* tiles = pk.at<GemmTile>(2);
-        launch_zero(D.stream, Sp.get(), Sp.bytes());  // blocks with fewer chunks leave parts unset
-        launch_gemm<double2>(D.stream, 0, 2, V.as<double2>(), V.as<double2>(), Sp.as<double2>(), tiles,
+        launch_zero(st, Sp.get(), Sp.bytes());  // blocks with fewer chunks leave parts unset
+        launch_gemm<double2>(st, 0, 2, V.as<double2>(), V.as<double2>(), Sp.as<double2>(), tiles,
                              s_tiles, pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0, 0.0);
-        launch_sum_parts(D.stream, Sp.as<double2>(), nch_max, (int64_t)nblk * sbw, S.as<double2>());
-        launch_hh_tfactor(D.stream, S.as<double2>(), X.tau.as<double2>(), T.as<double2>(), nblk, bw, nref);
+        launch_sum_parts(st, Sp.as<double2>(), nch_max, (int64_t)nblk * sbw, S.as<double2>());
+        launch_hh_tfactor(st, S.as<double2>(), X.tau.as<double2>(), T.as<double2>(), nblk, bw, nref);
         auto* v = reinterpret_cast<rocblas_double_complex*>(V.get());
         auto* p = reinterpret_cast<rocblas_double_complex*>(P.get());
         auto* t = reinterpret_cast<rocblas_double_complex*>(T.get());
@@ -688,19 +693,29 @@ inline void gram_eigvecs(Device& D, Solver& sol, SectorSvd& X, int lo) {
             KV_REQUIRE(st == rocblas_status_success, ERR_SOLVER, "rocBLAS zgemm failed");
         };
         // P_b = V_b T_b for every block at once
-        ok(rocblas_zgemm_strided_batched(sol.h, rocblas_operation_none, rocblas_operation_none, n, bw,
+        ok(rocblas_zgemm_strided_batched(h, rocblas_operation_none, rocblas_operation_none, n, bw,
                                          bw, &one, v, n, sv, t, bw, (rocblas_stride)sbw, &zero, p, n, sv,
                                          nblk));
         for (int b = nblk - 1; b >= 0; --b) {
             const int r0 = b * bw + 1;
             const int mb = n - r0;
             if (mb <= 0) continue;
-            launch_gemm<double2>(D.stream, 0, 2, Z.as<double2>(), V.as<double2>(), Yp.as<double2>(),
+            launch_gemm<double2>(st, 0, 2, Z.as<double2>(), V.as<double2>(), Yp.as<double2>(),
                                  tiles + y_off[b], y_cnt[b], pk.at<GemmGroup>(0), pk.at<GemmSeg>(1), 1.0,
                                  0.0);
-            launch_sum_parts(D.stream, Yp.as<double2>(), nch[b], (int64_t)bw * k, Y.as<double2>());
-            ok(rocblas_zgemm(sol.h, rocblas_operation_none, rocblas_operation_none, mb, k, bw, &mone,
+            launch_sum_parts(st, Yp.as<double2>(), nch[b], (int64_t)bw * k, Y.as<double2>());
+            ok(rocblas_zgemm(h, rocblas_operation_none, rocblas_operation_none, mb, k, bw, &mone,
                              p + (size_t)b * sv + r0, n, y, bw, &one, z + r0, n));
+        }
+        if (keep) {
+            keep->push_back(std::move(V));
+            keep->push_back(std::move(P));
+            keep->push_back(std::move(S));
+            keep->push_back(std::move(T));
+            keep->push_back(std::move(Y));
+            keep->push_back(std::move(Sp));
+            keep->push_back(std::move(Yp));
+            keep->push_back(std::move(pk.buf));
         }
     }
     X.Ev = std::move(Z);
@@ -1176,7 +1191,21 @@ inline void block_ritz(Device& D, Solver& sol, std::vector<BlockJob*>& J, bool i
         // the split route (one-workgroup tridiagonalisation, dstedc, back-transformation):
         // rocSOLVER's zheevd spends most of its time reducing a 256-wide matrix column by column
         gram_solve(D, sol, hp, mode == EIG_HEEVD ? EIG_HEEVD : EIG_SPLIT, roc_trd, true);
-        for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
+        if (hp.size() == 2 && mode != EIG_HEEVD) {
+            // the pair's back-transformations side by side: the second sector first, on the
+            // second stream (its temporaries are allocated before the first sector's and kept
+            // until the join), then the first on the context stream
+            std::vector<DevBuf> keep;
+            sol.second();
+            KV_HIP(hipEventRecord(sol.fork, D.stream));
+            KV_HIP(hipStreamWaitEvent(sol.s2, sol.fork, 0));
+            gram_eigvecs(D, sol, *hp[1], 0, sol.s2, sol.h2, &keep);
+            gram_eigvecs(D, sol, *hp[0], 0);
+            KV_HIP(hipEventRecord(sol.join, sol.s2));
+            KV_HIP(hipStreamWaitEvent(D.stream, sol.join, 0));
+        } else {
+            for (auto* h : hp) gram_eigvecs(D, sol, *h, 0);
+        }
         D.spans.close(t0, D.stream, &D.blk.eig_ms);
         D.blk.eig_calls += 1;
     }

@@ -92,7 +92,7 @@ struct PackedUpload {
 };
 
 template <class... V>
-inline PackedUpload upload_packed(Device& dev, const V&... vs) {
+inline PackedUpload upload_packed_on(Device& dev, hipStream_t on, const V&... vs) {
     PackedUpload p;
     size_t total = 0;
     auto place = [&](const auto& v) {
@@ -111,8 +111,13 @@ inline PackedUpload upload_packed(Device& dev, const V&... vs) {
     };
     (fill(vs), ...);
     p.buf = DevBuf(dev.pool.get(), total);
-    dev.upload(p.buf.get(), h.data(), total);
+    dev.upload(p.buf.get(), h.data(), total, on);
     return p;
+}
+
+template <class... V>
+inline PackedUpload upload_packed(Device& dev, const V&... vs) {
+    return upload_packed_on(dev, nullptr, vs...);
 }
 
 }  // namespace kv

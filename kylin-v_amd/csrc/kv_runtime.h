@@ -265,17 +265,21 @@ struct Device {
     // Host-to-device copy of a small host array (work descriptors) that returns without
     // waiting: the bytes go through a pinned staging ring, whose slots are reused only after
     // the next sync().  Copies larger than a quarter of the ring wait for themselves.
-    void upload(void* dst, const void* src, size_t bytes) {
+    // `on`: the stream the copy is ordered on (default the context stream); work on another
+    // stream must be joined back into the context stream before its next sync(), which
+    // restarts the ring.
+    void upload(void* dst, const void* src, size_t bytes, hipStream_t on = nullptr) {
         if (!bytes) return;
+        if (!on) on = stream;
         if (bytes > kRingBytes / 4) {
-            KV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
-            sync();
+            KV_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, on));
+            KV_HIP(hipStreamSynchronize(on));
             return;
         }
         if (ring_off_ + bytes > kRingBytes) sync();
         char* slot = static_cast<char*>(ring_) + ring_off_;
         std::memcpy(slot, src, bytes);
-        KV_HIP(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, stream));
+        KV_HIP(hipMemcpyAsync(dst, slot, bytes, hipMemcpyHostToDevice, on));
         ring_off_ = (ring_off_ + bytes + 255) / 256 * 256;
     }
 
