@@ -582,6 +582,32 @@ __global__ __launch_bounds__(256) void wfold_kernel(const T2* __restrict__ in,
     const WTerm* __restrict__ tt = terms + tk.term_begin;
     const int nt = tk.term_count, Q = tk.Q;
     const int PQ = tk.P * Q;
+    if constexpr (sizeof(T2) == 8) {
+        // FP32 complex: two neighbouring q per thread as one 16-byte load per term, when every
+        // offset and stride of the task is even (the addresses are then 16-byte aligned)
+        int64_t odd = tk.out_base | tk.out_pstride | (int64_t)Q;
+        for (int t = 0; t < nt; ++t) odd |= tt[t].in_base | tt[t].in_pstride;
+        if ((odd & 1) == 0) {
+            const int Q2 = Q >> 1, PQ2 = PQ >> 1;
+            for (int e = blockIdx.y * 256 + threadIdx.x; e < PQ2; e += gridDim.y * 256) {
+                const int p = e / Q2, q = 2 * (e - p * Q2);
+                float re0 = 0.f, im0 = 0.f, re1 = 0.f, im1 = 0.f;
+#pragma unroll 4
+                for (int t = 0; t < nt; ++t) {
+                    const WTerm w = tt[t];
+                    const float4 v = *reinterpret_cast<const float4*>(in + w.in_base + (int64_t)p * w.in_pstride + q);
+                    const float wr = (float)w.re, wi = (float)w.im;
+                    re0 += wr * v.x - wi * v.y;
+                    im0 += wr * v.y + wi * v.x;
+                    re1 += wr * v.z - wi * v.w;
+                    im1 += wr * v.w + wi * v.z;
+                }
+                *reinterpret_cast<float4*>(out + tk.out_base + (int64_t)p * tk.out_pstride + q) =
+                    make_float4(re0, im0, re1, im1);
+            }
+            return;
+        }
+    }
     for (int e = blockIdx.y * 256 + threadIdx.x; e < PQ; e += gridDim.y * 256) {
         const int p = e / Q, q = e - p * Q;
         T re = 0, im = 0;
