@@ -94,11 +94,111 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
     // behind the MFMAs of the same workgroup.
     constexpr int NA = (BM * BK) / 256, NB = (BN * BK) / 256;
     T2 ra[NA], rb[NB];
+    // FP32 complex operands load two neighbours along each operand's contiguous index per
+    // thread (one 16-byte access when the segment's offset and leading dimension are even).
+    constexpr bool kPair = sizeof(T2) == 8;
     auto load = [&](int s, int k0) {
         const GemmSeg sg = segs[g.seg_begin + s];
         const T2* __restrict__ a = A + sg.a_off;
         const T2* __restrict__ b = B + sg.b_off;
         const int K = sg.k;
+        if constexpr (kPair) {
+            const bool alA = ((sg.a_off | (int64_t)sg.lda) & 1) == 0;
+            const bool alB = ((sg.b_off | (int64_t)sg.ldb) & 1) == 0;
+#pragma unroll
+            for (int r = 0; r < NA / 2; ++r) {
+                const int e = tid + r * 256;
+                const int mm = (OPA == 0) ? e / (BK / 2) : 2 * (e % (BM / 2));
+                const int kk = (OPA == 0) ? 2 * (e % (BK / 2)) : e / (BM / 2);
+                const int gm = m0 + mm, gk = k0 + kk;
+                T2 v0, v1;
+                v0.x = 0;
+                v0.y = 0;
+                v1 = v0;
+                if (OPA == 0) {  // neighbours along k
+                    if (gm < g.m) {
+                        const T2* pa = a + (int64_t)gm * sg.lda + gk;
+                        if (alA && gk + 1 < K) {
+                            const float4 q = *reinterpret_cast<const float4*>(pa);
+                            v0.x = q.x;
+                            v0.y = q.y;
+                            v1.x = q.z;
+                            v1.y = q.w;
+                        } else {
+                            if (gk < K) v0 = pa[0];
+                            if (gk + 1 < K) v1 = pa[1];
+                        }
+                    }
+                } else {  // neighbours along m
+                    if (gk < K) {
+                        const T2* pa = a + (int64_t)gk * sg.lda + gm;
+                        if (alA && gm + 1 < g.m) {
+                            const float4 q = *reinterpret_cast<const float4*>(pa);
+                            v0.x = q.x;
+                            v0.y = q.y;
+                            v1.x = q.z;
+                            v1.y = q.w;
+                        } else {
+                            if (gm < g.m) v0 = pa[0];
+                            if (gm + 1 < g.m) v1 = pa[1];
+                        }
+                    }
+                }
+                if (OPA == 2) {
+                    v0.y = -v0.y;
+                    v1.y = -v1.y;
+                }
+                ra[2 * r] = v0;
+                ra[2 * r + 1] = v1;
+            }
+#pragma unroll
+            for (int r = 0; r < NB / 2; ++r) {
+                const int e = tid + r * 256;
+                const int nn = (OPB == 0) ? 2 * (e % (BN / 2)) : e / (BK / 2);
+                const int kk = (OPB == 0) ? e / (BN / 2) : 2 * (e % (BK / 2));
+                const int gn = n0 + nn, gk = k0 + kk;
+                T2 v0, v1;
+                v0.x = 0;
+                v0.y = 0;
+                v1 = v0;
+                if (OPB == 0) {  // neighbours along n
+                    if (gk < K) {
+                        const T2* pb = b + (int64_t)gk * sg.ldb + gn;
+                        if (alB && gn + 1 < g.n) {
+                            const float4 q = *reinterpret_cast<const float4*>(pb);
+                            v0.x = q.x;
+                            v0.y = q.y;
+                            v1.x = q.z;
+                            v1.y = q.w;
+                        } else {
+                            if (gn < g.n) v0 = pb[0];
+                            if (gn + 1 < g.n) v1 = pb[1];
+                        }
+                    }
+                } else {  // neighbours along k
+                    if (gn < g.n) {
+                        const T2* pb = b + (int64_t)gn * sg.ldb + gk;
+                        if (alB && gk + 1 < K) {
+                            const float4 q = *reinterpret_cast<const float4*>(pb);
+                            v0.x = q.x;
+                            v0.y = q.y;
+                            v1.x = q.z;
+                            v1.y = q.w;
+                        } else {
+                            if (gk < K) v0 = pb[0];
+                            if (gk + 1 < K) v1 = pb[1];
+                        }
+                    }
+                }
+                if (OPB == 2) {
+                    v0.y = -v0.y;
+                    v1.y = -v1.y;
+                }
+                rb[2 * r] = v0;
+                rb[2 * r + 1] = v1;
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + r * 256;
@@ -143,6 +243,31 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(BM * BN <= 
         }
     };
     auto store = [&](int buf) {
+        if constexpr (kPair) {
+#pragma unroll
+            for (int r = 0; r < NA / 2; ++r) {
+                const int e = tid + r * 256;
+                const int mm = (OPA == 0) ? e / (BK / 2) : 2 * (e % (BM / 2));
+                const int kk = (OPA == 0) ? 2 * (e % (BK / 2)) : e / (BM / 2);
+                As[buf][kk][mm] = ra[2 * r];
+                if (OPA == 0)
+                    As[buf][kk + 1][mm] = ra[2 * r + 1];
+                else
+                    As[buf][kk][mm + 1] = ra[2 * r + 1];
+            }
+#pragma unroll
+            for (int r = 0; r < NB / 2; ++r) {
+                const int e = tid + r * 256;
+                const int nn = (OPB == 0) ? 2 * (e % (BN / 2)) : e / (BK / 2);
+                const int kk = (OPB == 0) ? e / (BN / 2) : 2 * (e % (BK / 2));
+                Bs[buf][kk][nn] = rb[2 * r];
+                if (OPB == 0)
+                    Bs[buf][kk][nn + 1] = rb[2 * r + 1];
+                else
+                    Bs[buf][kk + 1][nn] = rb[2 * r + 1];
+            }
+            return;
+        }
 #pragma unroll
         for (int r = 0; r < NA; ++r) {
             const int e = tid + r * 256;
