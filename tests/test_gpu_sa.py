@@ -132,3 +132,30 @@ def test_sa_rejects_missing_widths():
         d.sa_one_site([1.0], 0)
     with pytest.raises(kv.KVError):
         d.sa_one_site([1.0, -2.0], 0)
+
+
+def test_sa_sweeps_in_mixed_precision_follow_fp64():
+    """The SA step with FP32 H_eff operands (FP64 enrichment SVD, Lanczos scalars and
+    environments) against the FP64 SA step from the same state, widths and seeds: the
+    states agree to the FP32 contraction error."""
+    print("case sa mixed precision", flush=True)
+    kv = GE.load_package()
+    N, Ls, om, J, dt = 4, [3], [1.0], 1.0, 0.05
+    g = np.array([0.9, 1.1, 0.8, 1.0]).reshape(4, 1)
+    lat = M.holstein(N, om, g, Ls, J, "merged")
+    mpo = M.build_mpo(lat)
+    psi0 = M.product_state(lat)
+    ref = T.TDVP(psi0, mpo, cutoff=1e-10, maxdim=6, dt=dt)
+    for _ in range(6):
+        ref.two_site()
+    betas = [1.0 / 0.3] * (N - 1)
+    runs = {}
+    for prec in ("fp64", "fp32"):
+        d = kv.TDVP(to_gpu(kv, ref.mps), to_gpu(kv, mpo), cutoff=1e-10, maxdim=40, dt=dt, precision=prec)
+        for k in range(3):
+            d.sa_one_site(betas, 11 + k)
+        runs[prec] = (d.bond_dims(), to_oracle(d.mps), d.stats()["sa_added"])
+    (d64, m64, a64), (d32, m32, a32) = runs["fp64"], runs["fp32"]
+    assert a64 > 0 and a32 > 0
+    assert abs(T.overlap_states(psi0, m64) - T.overlap_states(psi0, m32)) < 1e-5
+    assert 1 - abs(T.overlap_states(m64, m32)) < 1e-5
